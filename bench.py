@@ -1,0 +1,187 @@
+"""Benchmark: explicit MLS-MPM substeps on one MI355X.
+
+Workload (the headline configuration): 2^20 particles in three seeded cubes of
+side 0.2 (elastic fixed-corotated E=5e3 nu=0.2; snow E=5e3 nu=0.2 h=10,
+theta_c=2.5e-2, theta_s=7.5e-3; weakly compressible water K=1e4), one
+N(0, 0.5^2) velocity per cube, 128^3 grid on [0,1]^3, dt=5e-5, gravity -9.8,
+FP32, the reference's 3D walls.  A step is one substep of the whole scene
+(P2G, grid update, G2P with the constitutive and F update), with the state
+resident in HBM.
+
+Prints one JSON line.  `value` = particle-substeps/s over the timed steps.
+`roofline` is for the dominant kernel (the fused G2P+P2G transfer): its
+algorithmic bytes (212 B per particle-substep plus 28 B per active grid node)
+over its mean duration, measured with HIP events on the engine's stream.
+`cpu_baseline` times the C restatement of the reference step (FP64, one
+thread, the reference's loop order) on a bounded sample of the same scene.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MPM particle-substeps/s (1M p, 128³ grid), %HBM peak; corrector speedup"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_PARTICLE = 212.0
+BYTES_PER_NODE_TRANSFER = 28.0  # G2P reads 12 B of velocity + P2G writes 16 B per active node
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--res", type=int, default=128)
+    ap.add_argument("--chunk", type=int, default=512)
+    ap.add_argument("--sort-every", type=int, default=8)
+    ap.add_argument("--grid-mult", type=int, default=4)
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="budget of the CPU-baseline sample (0 skips it)")
+    ap.add_argument("--profile-steps", type=int, default=64,
+                    help="substeps run with per-kernel HIP-event timing")
+    return ap.parse_args()
+
+
+def cpu_baseline(state, params, budget_s):
+    """The C restatement of the reference step on the host (one thread)."""
+    from oracle import c_oracle
+
+    cs = c_oracle.CState(state)
+    t0 = time.perf_counter()
+    c_oracle.run(state, params, 1, nthreads=1, cstate=cs)
+    t1 = time.perf_counter() - t0
+    steps = 1
+    extra = int(max(0, min(9, budget_s / max(t1, 1e-6) - 1)))
+    if extra:
+        t0 = time.perf_counter()
+        c_oracle.run(state, params, extra, nthreads=1, cstate=cs)
+        t1 += time.perf_counter() - t0
+        steps += extra
+    n = len(state["x"])
+    return {"value": n * steps / t1, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
+            "sample": f"C restatement of the reference step (FP64, one thread, reference loop order), "
+                      f"{steps} substep(s) of the full {n}-particle scene from t=0"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); "
+              "this build has no multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    from femflow_amd.engine import Engine, MaterialSpec, SceneSpec
+    from femflow_amd.scenes import MATERIALS_C2, scene_3d_cubes
+
+    dev = torch.device("cuda", 0)
+    st, pr = scene_3d_cubes(n=args.n, res=args.res)
+    n = len(st["x"])
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float32, device=dev)  # noqa: E731
+    x, v, F, C, Jp = t(st["x"]), t(st["v"]), t(st["F"]), t(st["C"]), t(st["Jp"].reshape(n))
+    mass, mu0, lam0 = t(st["mass"]), t(st["mu0"]), t(st["lam0"])
+    kind = torch.tensor(st["kind"], dtype=torch.int32, device=dev)
+    mats = [MaterialSpec(("elastic", "snow", "water")[k], h, tc, ts) for k, _, _, h, tc, ts in MATERIALS_C2]
+    eng = Engine([SceneSpec(res=pr["res"], dt=pr["dt"], gravity=pr["gravity"], volume=pr["volume"])], mats,
+                 n_max=n, device=dev)
+    eng.tune(chunk=args.chunk, sort_every=args.sort_every, grid_mult=args.grid_mult,
+             use_graph=not args.no_graph)
+    eng.upload(x, v, F, C, Jp, mass, mu0, lam0, kind)
+    eng.run(args.warmup, check_oob=False)
+    torch.cuda.synchronize()
+    print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
+
+    # ---- timed region: exactly `steps` substeps --------------------------
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    eng.run(args.steps, check_oob=False)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    ms = e0.elapsed_time(e1)
+    oob = eng.oob_count()
+    if oob:
+        raise RuntimeError(f"{oob} particle stencils left the grid during the benchmark")
+    value = n * args.steps / (ms / 1e3)
+    print(f"[bench] timed {args.steps} substeps: {ms:.2f} ms (wall {wall * 1e3:.2f} ms)",
+          file=sys.stderr, flush=True)
+
+    # ---- per-kernel durations (HIP events on the engine's stream) --------
+    eng.kernel_timing(True)
+    nprof = max(args.profile_steps, args.sort_every)
+    sums = torch.zeros((nprof, 8), dtype=torch.float64, device=dev)
+    eng.run(nprof, sums=sums, check_oob=False)
+    kt = eng.kernel_times()
+    eng.kernel_timing(False)
+    g_ms, g_cnt = kt["g2p2g"]
+    dom = "g2p2g" if g_cnt else "p2g"
+    d_ms, d_cnt = kt[dom]
+    avg_s = d_ms / 1e3 / max(d_cnt, 1)
+    active_nodes = None
+    try:
+        active_nodes = int(eng.active_block_count()) * 64
+    except AttributeError:
+        pass
+    alg_bytes = BYTES_PER_PARTICLE * n + BYTES_PER_NODE_TRANSFER * (active_nodes or 0)
+    achieved = alg_bytes / avg_s / 1e9
+    s = sums.cpu().numpy()
+    mass_drift = float(np.abs(s[:, 0] / s[0, 0] - 1).max()) if nprof else 0.0
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "g2p2g_c2_1M_128_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(st, pr, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D MLS-MPM substep, 2^20 particles in three seeded cubes (elastic fixed-corotated, "
+                        "snow, weakly compressible water), 128^3 grid, dt=5e-5",
+            "particles": n, "grid": f"{args.res + 1}^3 nodes", "chunk": args.chunk,
+            "sort_every": args.sort_every, "hip_graph": not args.no_graph,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": dom, "kernel_ms": avg_s * 1e3,
+            "alg_bytes_per_launch": alg_bytes,
+        },
+        "step_hbm_frac": (BYTES_PER_PARTICLE * n / (ms / 1e3 / args.steps)) / 1e9 / HBM_PEAK_GBS,
+        "kernel_ms_per_step": {k: (vv[0] / vv[1] if vv[1] else 0.0) for k, vv in kt.items()},
+        "grid_mass_drift_fp64": mass_drift,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    eng.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,82 @@
+// Host-side engine object behind the mpm_ctx handle of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mpm355.h"
+#include "mpm_engine_kernels.h"
+
+namespace mpm {
+
+void set_error(const std::string& msg);
+
+struct GraphKey {
+  int len, parity;
+  uintptr_t sums;
+  int nsteps;
+  bool operator<(const GraphKey& o) const {
+    return std::tie(len, parity, sums, nsteps) < std::tie(o.len, o.parity, o.sums, o.nsteps);
+  }
+};
+
+struct TimedLaunch {
+  int kind;  // 0 P2G, 1 G2P2G, 2 G2P, 3 grid update
+  hipEvent_t e0, e1;
+};
+
+struct Engine {
+  // configuration
+  int device = 0, num_cus = 256;
+  int64_t n_max = 0, n = 0, total_blocks = 0;
+  int nmat = 1;
+  int chunk = 512, sort_every = 8, grid_mult = 4;
+  bool use_graph = true, timing = false;
+  std::vector<SceneDev> scenes_host;
+  // streams and graphs
+  hipStream_t user_stream = nullptr, own_stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  std::map<GraphKey, hipGraphExec_t> graphs;
+  std::vector<hipEvent_t> tevents;
+  size_t tnext = 0;
+  std::vector<TimedLaunch> tpairs;
+  // device memory
+  std::vector<void*> allocs;
+  EngineState st[2] = {};
+  int cur = 0;
+  SceneDev* d_scenes = nullptr;
+  MatParams* d_mats = nullptr;
+  int* d_block_scene = nullptr;
+  float4* d_acc = nullptr;
+  float4* d_vel = nullptr;
+  uint32_t *d_flags = nullptr, *d_list = nullptr, *d_bstart = nullptr, *d_bend = nullptr;
+  uint32_t* d_counters = nullptr;
+  unsigned long long *d_oob = nullptr, *d_fallback = nullptr;
+  int4* d_chunks = nullptr;
+  uint32_t *d_nchunks = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_cflags = nullptr,
+           *d_cidx = nullptr, *d_part = nullptr;
+  void* d_ws = nullptr;
+  size_t ws_bytes = 0;
+
+  ~Engine();
+  void release();
+  int alloc(void** p, size_t bytes);
+  int init(int64_t n_max, int nscenes, const mpm_scene* sc, int nmat, const mpm_material* mats);
+  EngineArgs args(double* sums, int max_steps) const;
+  int upload(const mpm_particles* p, const int32_t* scene_ids);
+  int download(mpm_particles* p);
+  void enqueue_bin(hipStream_t s);
+  void enqueue_segment(int len, hipStream_t s, double* sums, int max_steps);
+  void launch_transfer(int mode, hipStream_t s, const EngineArgs& a);
+  void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums);
+  hipEvent_t tevent();
+  int run(int nsteps, double* sums);
+  int take_counter(int which, int64_t* out);
+  int particle_sums(double* out);
+  int kernel_times(double* ms, int64_t* counts);
+};
+
+}  // namespace mpm

@@ -1,0 +1,956 @@
+// The particle-resident MLS-MPM engine: binning, LDS-arena P2G, the fused
+// G2P(n)+P2G(n+1) transfer, block-sparse grid update, and the rollout driver
+// (eager or replayed as a captured hipGraph).  Reference step semantics are
+// femflow/solvers/mpm/mls_mpm.py:40-79 with three_d/{p2g,grid_op,g2p}.py.
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "mpm_engine.h"
+#include "mpm_engine_kernels.h"
+#include "mpm_sort.h"
+
+namespace mpm {
+
+// ------------------------------------------------------------------ G2P ----
+// v = sum w g; C = 4 inv_dx sum w g (ijk - fx)^T  (three_d/g2p.py:29-46),
+// accumulated as C[:, a] = 4 inv_dx (sum w g ijk_a - fx_a v).
+template <bool FROM_LDS>
+__device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int abase,
+                                       const float4* __restrict__ vel, const SceneDev& sc,
+                                       const int base[3], const float w[3][3], const float fx[3],
+                                       float v[3], float C[9]) {
+  float S[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    v[a] = 0.f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) S[a][d] = 0.f;
+  }
+  size_t X[3], Y[3], Z[3];
+  if (!FROM_LDS) {
+    const size_t nb = (size_t)sc.nb;
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+      const int ni = base[0] + o, nj = base[1] + o, nk = base[2] + o;
+      X[o] = (size_t)(ni >> 2) * nb * nb * 64 + (size_t)((ni & 3) << 4);
+      Y[o] = (size_t)(nj >> 2) * nb * 64 + (size_t)((nj & 3) << 2);
+      Z[o] = (size_t)(nk >> 2) * 64 + (size_t)(nk & 3);
+    }
+  }
+  const size_t gbase = (size_t)sc.block_base * 64;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wij = w[0][i] * w[1][j];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        float4 g;
+        if (FROM_LDS) g = vel_s[abase + i * 64 + j * 8 + k];
+        else g = vel[gbase + X[i] + Y[j] + Z[k]];
+        const float wt = wij * w[2][k];
+        const float wg[3] = {wt * g.x, wt * g.y, wt * g.z};
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          v[d] += wg[d];
+          if (i) S[0][d] += (float)i * wg[d];
+          if (j) S[1][d] += (float)j * wg[d];
+          if (k) S[2][d] += (float)k * wg[d];
+        }
+      }
+    }
+  const float c4 = 4.f * sc.inv_dx;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int a = 0; a < 3; ++a) C[3 * d + a] = c4 * (S[a][d] - fx[a] * v[d]);
+}
+
+// x += dt v, F <- (I + dt C) F, then plasticity (three_d/g2p.py:44-58).
+__device__ __forceinline__ int advance(const SceneDev& sc, const MatParams& mp, float x[3],
+                                       const float v[3], const float C[9], float F[9], float& Jp) {
+  int fb = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) x[d] += sc.dt * v[d];
+  float M[9], Fn[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) M[i] = sc.dt * C[i] + ((i % 4 == 0) ? 1.f : 0.f);
+  matmul3(M, F, Fn);
+  if (mp.kind == MPM_SNOW) {
+    float U[9], s[3], V[9];
+    fb = svd3_checked<float>(Fn, U, s, V);
+    const float oldJ = det3(Fn);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s[c] = tmin(tmax(s[c], 1.f - mp.theta_c), 1.f + mp.theta_s);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        Fn[3 * i + j] = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
+                        U[3 * i + 2] * s[2] * V[3 * j + 2];
+    const float det = det3(Fn) + 1e-10f;
+    Jp = tmin(tmax(Jp * oldJ / det, 0.6f), 20.f);
+  } else if (mp.kind == MPM_WATER) {
+    const float c = tcbrt(det3(Fn));
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Fn[i] = (i % 4 == 0) ? c : 0.f;
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) F[i] = Fn[i];
+  return fb;
+}
+
+// ------------------------------------------------------------------ P2G ----
+// Per node (i,j,k): w_ijk (m v + A (ijk - fx) dx) and w_ijk m (three_d/p2g.py:67-80),
+// written as w (q0 + Adx ijk) with q0 = m v - A fx dx, Adx = A dx.
+struct P2GTerms {
+  float q0[3];
+  float Adx[9];
+  float m;
+};
+
+__device__ __forceinline__ int p2g_terms(const SceneDev& sc, const MatParams& mp, float mass,
+                                         float mu0, float lam0, float Jp, const float F[9],
+                                         const float C[9], const float v[3], const float fx[3],
+                                         P2GTerms& t) {
+  float mu, lam, A[9];
+  lame<float>(mp.kind, mp.hardening, mu0, lam0, Jp, mu, lam);
+  const int fb = affine3<float>(F, C, mass, mu, lam, mp.kind, sc.inv_dx, sc.dt, sc.vol,
+                                sc.quirks == MPM_QUIRKS_TEXTBOOK, A);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    t.q0[d] = mass * v[d] - sc.dx * (A[3 * d] * fx[0] + A[3 * d + 1] * fx[1] + A[3 * d + 2] * fx[2]);
+#pragma unroll
+    for (int e = 0; e < 3; ++e) t.Adx[3 * d + e] = A[3 * d + e] * sc.dx;
+  }
+  t.m = mass;
+  return fb;
+}
+
+__device__ __forceinline__ void scatter_lds(float (*acc)[ARENA_NODES], int abase, const float w[3][3],
+                                            const P2GTerms& t) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wij = w[0][i] * w[1][j];
+      float qij[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) qij[d] = t.q0[d] + t.Adx[3 * d] * (float)i + t.Adx[3 * d + 1] * (float)j;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float wt = wij * w[2][k];
+        const int off = abase + i * 64 + j * 8 + k;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) atomicAdd(&acc[d][off], wt * (qij[d] + t.Adx[3 * d + 2] * (float)k));
+        atomicAdd(&acc[3][off], wt * t.m);
+      }
+    }
+}
+
+__device__ __forceinline__ void scatter_global(const EngineArgs& a, const SceneDev& sc,
+                                               const int base[3], const float w[3][3],
+                                               const P2GTerms& t) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float wt = w[0][i] * w[1][j] * w[2][k];
+        const size_t n = node_addr(sc, base[0] + i, base[1] + j, base[2] + k);
+        float* dst = reinterpret_cast<float*>(a.acc + n);
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          atomicAdd(dst + d, wt * (t.q0[d] + t.Adx[3 * d] * (float)i + t.Adx[3 * d + 1] * (float)j +
+                                   t.Adx[3 * d + 2] * (float)k));
+        atomicAdd(dst + 3, wt * t.m);
+      }
+  // mark the (up to 8) blocks the stencil touched
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int i = base[0] + ((c >> 2) & 1) * 2, j = base[1] + ((c >> 1) & 1) * 2, k = base[2] + (c & 1) * 2;
+    const uint32_t g = (uint32_t)(sc.block_base + ((i >> 2) * sc.nb + (j >> 2)) * sc.nb + (k >> 2));
+    mark_active(a, g);
+  }
+}
+
+// ----------------------------------------------------------- particle I/O ----
+__device__ __forceinline__ void load3(const float* p, int64_t i, float o[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) o[d] = p[3 * i + d];
+}
+__device__ __forceinline__ void load9(const float* p, int64_t i, float o[9]) {
+#pragma unroll
+  for (int d = 0; d < 9; ++d) o[d] = p[9 * i + d];
+}
+__device__ __forceinline__ void store3(float* p, int64_t i, const float o[3]) {
+#pragma unroll
+  for (int d = 0; d < 3; ++d) p[3 * i + d] = o[d];
+}
+__device__ __forceinline__ void store9(float* p, int64_t i, const float o[9]) {
+#pragma unroll
+  for (int d = 0; d < 9; ++d) p[9 * i + d] = o[d];
+}
+
+// --------------------------------------------------------- transfer kernel ----
+// MODE 0: P2G only (after binning).  MODE 1: fused G2P + P2G.  MODE 2: G2P only.
+// Persistent: each XCD (workgroup id mod 8) walks a contiguous eighth of the
+// chunk list, so neighbouring chunks share that XCD's L2.
+template <int MODE>
+__global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
+  __shared__ float4 vel_s[MODE == 0 ? 1 : ARENA_NODES];
+  __shared__ float acc_s[4][MODE == 2 ? 1 : ARENA_NODES];
+  __shared__ uint32_t sub_active[27];
+  __shared__ uint32_t wg_oob, wg_fb;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wv = t >> 6;
+  if (t == 0) {
+    wg_oob = 0;
+    wg_fb = 0;
+  }
+  const uint32_t nch = *a.nchunks;
+  const int xcd = blockIdx.x & 7;
+  const int l = blockIdx.x >> 3;
+  const int L = ((int)gridDim.x - xcd + 7) >> 3;
+  const uint32_t lo = (uint32_t)(((uint64_t)nch * xcd) >> 3);
+  const uint32_t hi = (uint32_t)(((uint64_t)nch * (xcd + 1)) >> 3);
+  for (uint32_t c = lo + l; c < hi; c += L) {
+    const int4 ch = a.chunks[c];
+    const SceneDev sc = a.scenes[ch.w];
+    const int local = ch.z - sc.block_base;
+    const int nb = sc.nb;
+    const int bi = local / (nb * nb), bj = (local / nb) % nb, bk = local % nb;
+    const int O0 = bi * 4 - 1, O1 = bj * 4 - 1, O2 = bk * 4 - 1;
+    for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
+      if constexpr (MODE != 0) {
+        const int gi = O0 + (e >> 6), gj = O1 + ((e >> 3) & 7), gk = O2 + (e & 7);
+        const bool in = gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res;
+        vel_s[e] = in ? a.vel[node_addr(sc, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      if constexpr (MODE != 2) {
+        acc_s[0][e] = 0.f;
+        acc_s[1][e] = 0.f;
+        acc_s[2][e] = 0.f;
+        acc_s[3][e] = 0.f;
+      }
+    }
+    if (t < 27) sub_active[t] = 0;
+    __syncthreads();
+    // lanes of one wave take particles S apart so that lanes rarely share a
+    // cell (and hence the same LDS addresses) in one atomic instruction
+    const int S = (ch.y + 63) >> 6;
+    const int P = (S + 3) >> 2;
+    for (int r = 0; r < P; ++r) {
+      const int jj = wv * P + r;
+      const int q = lane * S + jj;
+      if (jj >= S || q >= ch.y) continue;
+      const int64_t p = (int64_t)ch.x + q;
+      float x[3], F[9], v[3], C[9], Jp;
+      load3(a.s.x, p, x);
+      load9(a.s.F, p, F);
+      Jp = a.s.Jp[p];
+      const uint32_t attr = a.s.attr[p];
+      int mid = (int)(attr & 255u);
+      mid = mid < a.nmat ? mid : a.nmat - 1;
+      const MatParams mp = a.mats[mid];
+      int base[3];
+      float fx[3], w[3][3];
+      if constexpr (MODE == 0) {
+        load3(a.s.v, p, v);
+        load9(a.s.C, p, C);
+      } else {
+        if (!locate3(x, sc, base, fx)) {
+          atomicAdd(&wg_oob, 1u);
+          continue;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
+        const int a0 = base[0] - O0, a1 = base[1] - O1, a2 = base[2] - O2;
+        if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5)
+          gather<true>(vel_s, (a0 * 8 + a1) * 8 + a2, a.vel, sc, base, w, fx, v, C);
+        else
+          gather<false>(vel_s, 0, a.vel, sc, base, w, fx, v, C);
+        const int fb = advance(sc, mp, x, v, C, F, Jp);
+        if (fb) atomicAdd(&wg_fb, 1u);
+        store3(a.s.x, p, x);
+        store3(a.s.v, p, v);
+        store9(a.s.C, p, C);
+        store9(a.s.F, p, F);
+        a.s.Jp[p] = Jp;
+      }
+      if constexpr (MODE != 2) {
+        if (!locate3(x, sc, base, fx)) {
+          atomicAdd(&wg_oob, 1u);
+          continue;
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
+        P2GTerms tm;
+        const int fb = p2g_terms(sc, mp, a.s.mass[p], a.s.mu0[p], a.s.lam0[p], Jp, F, C, v, fx, tm);
+        if (fb) atomicAdd(&wg_fb, 1u);
+        const int a0 = base[0] - O0, a1 = base[1] - O1, a2 = base[2] - O2;
+        if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5)
+          scatter_lds(acc_s, (a0 * 8 + a1) * 8 + a2, w, tm);
+        else
+          scatter_global(a, sc, base, w, tm);
+      }
+    }
+    if constexpr (MODE != 2) {
+      __syncthreads();
+      for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
+        const float m0 = acc_s[0][e], m1 = acc_s[1][e], m2 = acc_s[2][e], m3 = acc_s[3][e];
+        if (m0 != 0.f || m1 != 0.f || m2 != 0.f || m3 != 0.f) {
+          const int gi = O0 + (e >> 6), gj = O1 + ((e >> 3) & 7), gk = O2 + (e & 7);
+          if (gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res) {
+            float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, gi, gj, gk));
+            atomicAdd(dst + 0, m0);
+            atomicAdd(dst + 1, m1);
+            atomicAdd(dst + 2, m2);
+            atomicAdd(dst + 3, m3);
+            const int si = (gi >> 2) - (bi - 1), sj = (gj >> 2) - (bj - 1), sk = (gk >> 2) - (bk - 1);
+            sub_active[(si * 3 + sj) * 3 + sk] = 1u;
+          }
+        }
+      }
+      __syncthreads();
+      if (t < 27 && sub_active[t]) {
+        const int gbi = bi - 1 + t / 9, gbj = bj - 1 + (t / 3) % 3, gbk = bk - 1 + t % 3;
+        if (gbi >= 0 && gbj >= 0 && gbk >= 0 && gbi < nb && gbj < nb && gbk < nb)
+          mark_active(a, (uint32_t)(sc.block_base + (gbi * nb + gbj) * nb + gbk));
+      }
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  if (t == 0) {
+    if (wg_oob) atomicAdd(a.oob, (unsigned long long)wg_oob);
+    if (wg_fb) atomicAdd(a.fallback, (unsigned long long)wg_fb);
+  }
+}
+
+// ------------------------------------------------------------ grid update ----
+// three_d/grid_op.py:25-47 on the active blocks only; also zeroes the
+// accumulation grid and the block flags for the next P2G, and (optionally)
+// adds FP64 conservation sums for this substep.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+__global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, int with_sums) {
+  __shared__ double red[ENG_THREADS / 64][8];
+  const uint32_t count = a.counters[0];
+  const int t = threadIdx.x;
+  const int sub = t >> 6, inner = t & 63;
+  double s[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b = blockIdx.x * 4u + (uint32_t)sub; b < count; b += gridDim.x * 4u) {
+    const uint32_t g = a.list[b];
+    const SceneDev sc = a.scenes[a.block_scene[g]];
+    const int local = (int)g - sc.block_base;
+    const int nb = sc.nb;
+    const int i = (local / (nb * nb)) * 4 + (inner >> 4);
+    const int j = ((local / nb) % nb) * 4 + ((inner >> 2) & 3);
+    const int k = (local % nb) * 4 + (inner & 3);
+    const size_t addr = (size_t)g * 64 + inner;
+    const float4 q = a.acc[addr];
+    const float m = q.w;
+    float v[3] = {q.x, q.y, q.z};
+    if (with_sums) {
+      s[0] += m;
+      s[1] += q.x;
+      s[2] += q.y;
+      s[3] += q.z;
+    }
+    if (m > 0.f) {
+      const float im = 1.f / m;
+      v[0] *= im;
+      v[1] *= im;
+      v[2] *= im;
+      v[1] += sc.dt * sc.gravity;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = tmin(tmax(v[d], -sc.v_allowed), sc.v_allowed);
+    }
+    const int I[3] = {i, j, k};
+    if (sc.boundary == MPM_BC_REFERENCE) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (I[d] < 1 || I[d] >= sc.res - 1) v[d] = 0.f;
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (I[d] < sc.bound || I[d] > sc.res - sc.bound) v[0] = v[1] = v[2] = 0.f;
+    }
+    if (i > sc.res || j > sc.res || k > sc.res) v[0] = v[1] = v[2] = 0.f;
+    a.vel[addr] = make_float4(v[0], v[1], v[2], m);
+    a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (with_sums) {
+      s[4] += (double)m * v[0];
+      s[5] += (double)m * v[1];
+      s[6] += (double)m * v[2];
+    }
+    if (inner == 0) a.flags[g] = 0u;
+  }
+  if (with_sums) {
+#pragma unroll
+    for (int c = 0; c < 7; ++c) {
+      const double r = wave_sum(s[c]);
+      if ((t & 63) == 0) red[sub][c] = r;
+    }
+  }
+  __syncthreads();
+  if (t == 0) {
+    if (with_sums) {
+      const uint32_t step = a.counters[2];
+      if ((int)step < a.max_steps) {
+        for (int c = 0; c < 7; ++c) {
+          double r = 0;
+          for (int w = 0; w < ENG_THREADS / 64; ++w) r += red[w][c];
+          atomicAdd(a.sums + (size_t)step * 8 + c, r);
+        }
+      }
+    }
+    __threadfence();
+    const uint32_t prev = atomicAdd(a.counters + 1, 1u);
+    if (prev == gridDim.x - 1) {
+      a.counters[0] = 0u;
+      a.counters[1] = 0u;
+      if (with_sums) a.counters[2] += 1u;
+      __threadfence();
+    }
+  }
+}
+
+// ------------------------------------------------------- binning helpers ----
+__device__ __forceinline__ int cell_of(float x, float inv_dx, int res) {
+  const float t = __fsub_rn(__fmul_rn(x, inv_dx), 0.5f);
+  int b;
+  if (!(t > -1.0f)) b = 0;
+  else if (!(t < (float)res)) b = res;
+  else b = (int)t;
+  return b < 0 ? 0 : (b > res ? res : b);
+}
+
+__global__ void engine_keys_kernel(EngineArgs a, int64_t n, uint32_t* __restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SceneDev sc = a.scenes[a.s.attr[i] >> 8];
+  const int b0 = cell_of(a.s.x[3 * i], sc.inv_dx, sc.res);
+  const int b1 = cell_of(a.s.x[3 * i + 1], sc.inv_dx, sc.res);
+  const int b2 = cell_of(a.s.x[3 * i + 2], sc.inv_dx, sc.res);
+  const uint32_t blk = (uint32_t)(sc.block_base + ((b0 >> 2) * sc.nb + (b1 >> 2)) * sc.nb + (b2 >> 2));
+  keys[i] = blk * 64u + (uint32_t)(((b0 & 3) << 4) | ((b1 & 3) << 2) | (b2 & 3));
+}
+
+__global__ void reorder_kernel(EngineState src, EngineState dst, const uint32_t* __restrict__ perm,
+                               int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = perm[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    dst.x[3 * i + d] = src.x[3 * s + d];
+    dst.v[3 * i + d] = src.v[3 * s + d];
+  }
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    dst.F[9 * i + d] = src.F[9 * s + d];
+    dst.C[9 * i + d] = src.C[9 * s + d];
+  }
+  dst.Jp[i] = src.Jp[s];
+  dst.mass[i] = src.mass[s];
+  dst.mu0[i] = src.mu0[s];
+  dst.lam0[i] = src.lam0[s];
+  dst.attr[i] = src.attr[s];
+  dst.order[i] = src.order[s];
+}
+
+__global__ void block_bounds_kernel(const uint32_t* __restrict__ keys, int64_t n,
+                                    uint32_t* __restrict__ bstart, uint32_t* __restrict__ bend) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t blk = keys[i] >> 6;
+  if (i == 0 || (keys[i - 1] >> 6) != blk) bstart[blk] = (uint32_t)i;
+  if (i == n - 1 || (keys[i + 1] >> 6) != blk) bend[blk] = (uint32_t)(i + 1);
+}
+
+__global__ void chunk_flags_kernel(const uint32_t* __restrict__ keys, int64_t n,
+                                   const uint32_t* __restrict__ bstart, int chunk,
+                                   uint32_t* __restrict__ flags) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = (((uint32_t)i - bstart[keys[i] >> 6]) % (uint32_t)chunk) == 0u ? 1u : 0u;
+}
+
+__global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
+                                   const uint32_t* __restrict__ flags, const uint32_t* __restrict__ idx,
+                                   const uint32_t* __restrict__ bend, const int* __restrict__ block_scene,
+                                   int chunk, int4* __restrict__ chunks, uint32_t* __restrict__ nchunks) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i]) {
+    const uint32_t blk = keys[i] >> 6;
+    const uint32_t cnt = bend[blk] - (uint32_t)i;
+    chunks[idx[i]] = make_int4((int)i, (int)(cnt < (uint32_t)chunk ? cnt : (uint32_t)chunk), (int)blk,
+                               block_scene[blk]);
+  }
+  if (i == n - 1) *nchunks = idx[i] + flags[i];
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void fill_f64_kernel(double* p, int64_t n, double v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ----------------------------------------------------------- upload/download ----
+__global__ void upload_kernel(EngineState s, int64_t n, const float* __restrict__ x,
+                              const float* __restrict__ v, const float* __restrict__ F,
+                              const float* __restrict__ C, const float* __restrict__ Jp,
+                              const float* __restrict__ mass, const float* __restrict__ mu0,
+                              const float* __restrict__ lam0, int mat_scalar,
+                              const int32_t* __restrict__ material, const int32_t* __restrict__ scene,
+                              int nscenes) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    s.x[3 * i + d] = x[3 * i + d];
+    s.v[3 * i + d] = v[3 * i + d];
+  }
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    s.F[9 * i + d] = F[9 * i + d];
+    s.C[9 * i + d] = C[9 * i + d];
+  }
+  s.Jp[i] = Jp[i];
+  const int64_t mi = mat_scalar ? 0 : i;
+  s.mass[i] = mass[mi];
+  s.mu0[i] = mu0[mi];
+  s.lam0[i] = lam0[mi];
+  const uint32_t mat = material ? (uint32_t)(material[i] & 255) : 0u;
+  int sc_i = scene ? scene[i] : 0;
+  sc_i = sc_i < 0 ? 0 : (sc_i >= nscenes ? nscenes - 1 : sc_i);
+  const uint32_t sc = (uint32_t)sc_i;
+  s.attr[i] = mat | (sc << 8);
+  s.order[i] = (uint32_t)i;
+}
+
+__global__ void download_kernel(EngineState s, int64_t n, float* __restrict__ x, float* __restrict__ v,
+                                float* __restrict__ F, float* __restrict__ C, float* __restrict__ Jp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t o = s.order[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    x[3 * o + d] = s.x[3 * i + d];
+    v[3 * o + d] = s.v[3 * i + d];
+  }
+#pragma unroll
+  for (int d = 0; d < 9; ++d) {
+    F[9 * o + d] = s.F[9 * i + d];
+    C[9 * o + d] = s.C[9 * i + d];
+  }
+  Jp[o] = s.Jp[i];
+}
+
+__global__ void particle_sums_kernel(EngineState s, int64_t n, double* out) {
+  __shared__ double red[ENG_THREADS / 64][8];
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double m = s.mass[i];
+    acc[0] += m;
+    for (int d = 0; d < 3; ++d) {
+      acc[1 + d] += m * (double)s.v[3 * i + d];
+      acc[4 + d] += m * (double)s.x[3 * i + d];
+    }
+  }
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int c = 0; c < 7; ++c) {
+    const double r = wave_sum(acc[c]);
+    if ((t & 63) == 0) red[t >> 6][c] = r;
+  }
+  __syncthreads();
+  if (t < 7) {
+    double r = 0;
+    for (int w = 0; w < ENG_THREADS / 64; ++w) r += red[w][t];
+    atomicAdd(out + t, r);
+  }
+}
+
+// ================================================================ host ====
+static inline unsigned nblk(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+#define HIPCHK(expr)                                                                 \
+  do {                                                                               \
+    hipError_t _e = (expr);                                                          \
+    if (_e != hipSuccess) {                                                          \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+      return MPM_ERR_HIP;                                                            \
+    }                                                                                \
+  } while (0)
+
+Engine::~Engine() { release(); }
+
+void Engine::release() {
+  for (auto& kv : graphs) {
+    if (kv.second) hipGraphExecDestroy(kv.second);
+  }
+  graphs.clear();
+  for (void* p : allocs) hipFree(p);
+  allocs.clear();
+  if (ev_in) hipEventDestroy(ev_in);
+  if (ev_out) hipEventDestroy(ev_out);
+  for (auto& e : tevents) hipEventDestroy(e);
+  tevents.clear();
+  if (own_stream) hipStreamDestroy(own_stream);
+  ev_in = ev_out = nullptr;
+  own_stream = nullptr;
+}
+
+int Engine::alloc(void** p, size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    return MPM_ERR_HIP;
+  }
+  allocs.push_back(*p);
+  return MPM_OK;
+}
+
+int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, const mpm_material* mats) {
+  if (n_max_ < 0 || nscenes <= 0 || nscenes > MAX_SCENES || !sc || nmat_ <= 0 || nmat_ > 256 || !mats) {
+    set_error("mpm_ctx_create: bad argument (n_max >= 0, 1 <= nscenes <= 4096, 1 <= nmat <= 256)");
+    return MPM_ERR_ARG;
+  }
+  n_max = n_max_;
+  nmat = nmat_;
+  HIPCHK(hipGetDevice(&device));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, device));
+  num_cus = prop.multiProcessorCount;
+  HIPCHK(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&ev_in, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&ev_out, hipEventDisableTiming));
+  // scene table
+  std::vector<SceneDev> hs(nscenes);
+  std::vector<int> bscene;
+  int64_t blocks = 0;
+  for (int s = 0; s < nscenes; ++s) {
+    const mpm_scene& q = sc[s];
+    if (q.res < 4 || q.res > 1024 || !(q.dt > 0)) {
+      set_error("mpm_ctx_create: scene res must be in [4, 1024] and dt > 0");
+      return MPM_ERR_ARG;
+    }
+    SceneDev d{};
+    d.res = q.res;
+    d.nb = (q.res + 1 + 3) / 4;
+    d.block_base = (int)blocks;
+    d.boundary = q.boundary;
+    d.bound = q.bound;
+    d.quirks = q.quirks;
+    d.dx = (float)(1.0 / q.res);
+    d.inv_dx = (float)q.res;
+    d.dt = (float)q.dt;
+    d.gravity = (float)q.gravity;
+    d.vol = (float)q.volume;
+    d.v_allowed = (float)((1.0 / q.res) * 0.9 / q.dt);
+    hs[s] = d;
+    const int64_t nbs = (int64_t)d.nb * d.nb * d.nb;
+    bscene.insert(bscene.end(), (size_t)nbs, s);
+    blocks += nbs;
+  }
+  if (blocks * 64 >= (int64_t)1 << 32) {
+    set_error("mpm_ctx_create: total grid blocks exceed the 32-bit key space");
+    return MPM_ERR_ARG;
+  }
+  total_blocks = blocks;
+  scenes_host = hs;
+  int rc;
+  const int64_t n = n_max > 0 ? n_max : 1;
+  if ((rc = alloc((void**)&d_scenes, sizeof(SceneDev) * nscenes))) return rc;
+  if ((rc = alloc((void**)&d_mats, sizeof(MatParams) * nmat))) return rc;
+  if ((rc = alloc((void**)&d_block_scene, sizeof(int) * blocks))) return rc;
+  HIPCHK(hipMemcpy(d_scenes, hs.data(), sizeof(SceneDev) * nscenes, hipMemcpyHostToDevice));
+  std::vector<MatParams> hm(nmat);
+  for (int i = 0; i < nmat; ++i) {
+    hm[i].kind = mats[i].kind;
+    hm[i].hardening = mats[i].hardening;
+    hm[i].theta_c = mats[i].theta_c;
+    hm[i].theta_s = mats[i].theta_s;
+  }
+  HIPCHK(hipMemcpy(d_mats, hm.data(), sizeof(MatParams) * nmat, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(d_block_scene, bscene.data(), sizeof(int) * blocks, hipMemcpyHostToDevice));
+  // particle state (two copies for reordering)
+  for (int b = 0; b < 2; ++b) {
+    EngineState& s = st[b];
+    if ((rc = alloc((void**)&s.x, 12 * n)) || (rc = alloc((void**)&s.v, 12 * n)) ||
+        (rc = alloc((void**)&s.F, 36 * n)) || (rc = alloc((void**)&s.C, 36 * n)) ||
+        (rc = alloc((void**)&s.Jp, 4 * n)) || (rc = alloc((void**)&s.mass, 4 * n)) ||
+        (rc = alloc((void**)&s.mu0, 4 * n)) || (rc = alloc((void**)&s.lam0, 4 * n)) ||
+        (rc = alloc((void**)&s.attr, 4 * n)) || (rc = alloc((void**)&s.order, 4 * n)))
+      return rc;
+  }
+  // grids
+  if ((rc = alloc((void**)&d_acc, sizeof(float4) * 64 * blocks))) return rc;
+  if ((rc = alloc((void**)&d_vel, sizeof(float4) * 64 * blocks))) return rc;
+  if ((rc = alloc((void**)&d_flags, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_list, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_bstart, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_bend, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_counters, sizeof(uint32_t) * 16))) return rc;
+  if ((rc = alloc((void**)&d_oob, sizeof(unsigned long long) * 2))) return rc;
+  d_fallback = d_oob + 1;
+  // binning workspace
+  const int64_t max_chunks = n / 256 + blocks + 1;
+  if ((rc = alloc((void**)&d_chunks, sizeof(int4) * (max_chunks < n + 1 ? max_chunks : n + 1)))) return rc;
+  if ((rc = alloc((void**)&d_nchunks, sizeof(uint32_t) * 4))) return rc;
+  if ((rc = alloc((void**)&d_keys, sizeof(uint32_t) * n))) return rc;
+  if ((rc = alloc((void**)&d_vals, sizeof(uint32_t) * n))) return rc;
+  if ((rc = alloc((void**)&d_cflags, sizeof(uint32_t) * n))) return rc;
+  if ((rc = alloc((void**)&d_cidx, sizeof(uint32_t) * n))) return rc;
+  if ((rc = alloc((void**)&d_part, sizeof(uint32_t) * (n / 4096 + 4)))) return rc;
+  ws_bytes = radix_workspace_bytes(n);
+  if ((rc = alloc(&d_ws, ws_bytes))) return rc;
+  HIPCHK(hipMemset(d_acc, 0, sizeof(float4) * 64 * blocks));
+  HIPCHK(hipMemset(d_vel, 0, sizeof(float4) * 64 * blocks));
+  HIPCHK(hipMemset(d_flags, 0, sizeof(uint32_t) * blocks));
+  HIPCHK(hipMemset(d_counters, 0, sizeof(uint32_t) * 16));
+  HIPCHK(hipMemset(d_oob, 0, sizeof(unsigned long long) * 2));
+  HIPCHK(hipMemset(d_nchunks, 0, sizeof(uint32_t) * 4));
+  HIPCHK(hipDeviceSynchronize());
+  return MPM_OK;
+}
+
+EngineArgs Engine::args(double* sums, int max_steps) const {
+  EngineArgs a{};
+  a.s = st[cur];
+  a.scenes = d_scenes;
+  a.mats = d_mats;
+  a.nmat = nmat;
+  a.acc = d_acc;
+  a.vel = d_vel;
+  a.flags = d_flags;
+  a.list = d_list;
+  a.counters = d_counters;
+  a.chunks = d_chunks;
+  a.nchunks = d_nchunks;
+  a.block_scene = d_block_scene;
+  a.oob = d_oob;
+  a.fallback = d_fallback;
+  a.sums = sums;
+  a.max_steps = max_steps;
+  return a;
+}
+
+int Engine::upload(const mpm_particles* p, const int32_t* scene_ids) {
+  if (!p || p->dtype != MPM_F32 || p->n < 0 || p->n > n_max || !p->x || !p->v || !p->F || !p->C ||
+      !p->Jp || !p->mass || !p->mu0 || !p->lambda0) {
+    set_error("mpm_state_upload: needs FP32 arrays x, v, F, C, Jp, mass, mu0, lambda0 and n <= n_max");
+    return MPM_ERR_ARG;
+  }
+  if (nmat > 1 && !p->material) {
+    set_error("mpm_state_upload: material ids are required when the table has more than one entry");
+    return MPM_ERR_ARG;
+  }
+  n = p->n;
+  cur = 0;
+  if (n > 0)
+    hipLaunchKernelGGL(upload_kernel, dim3(nblk(n, 256)), dim3(256), 0, own_stream, st[0], n,
+                       (const float*)p->x, (const float*)p->v, (const float*)p->F, (const float*)p->C,
+                       (const float*)p->Jp, (const float*)p->mass, (const float*)p->mu0,
+                       (const float*)p->lambda0, p->mat_scalar, p->material, scene_ids,
+                       (int)scenes_host.size());
+  HIPCHK(hipGetLastError());
+  return MPM_OK;
+}
+
+int Engine::download(mpm_particles* p) {
+  if (!p || p->dtype != MPM_F32 || p->n != n || !p->x || !p->v || !p->F || !p->C || !p->Jp) {
+    set_error("mpm_state_download: needs FP32 arrays x, v, F, C, Jp with the uploaded n");
+    return MPM_ERR_ARG;
+  }
+  if (n > 0)
+    hipLaunchKernelGGL(download_kernel, dim3(nblk(n, 256)), dim3(256), 0, own_stream, st[cur], n,
+                       (float*)p->x, (float*)p->v, (float*)p->F, (float*)p->C, (float*)p->Jp);
+  HIPCHK(hipGetLastError());
+  return MPM_OK;
+}
+
+// Keys -> stable radix sort -> physical reorder -> chunk list.
+void Engine::enqueue_bin(hipStream_t s) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_nchunks, (int64_t)1, 0u);
+    return;
+  }
+  EngineArgs a = args(nullptr, 0);
+  hipLaunchKernelGGL(engine_keys_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, a, n, d_keys);
+  uint32_t *ks, *vs;
+  const uint32_t max_key = (uint32_t)(total_blocks * 64 - 1);
+  radix_sort(d_keys, d_vals, n, max_key, d_ws, true, &ks, &vs, s);
+  const int nxt = cur ^ 1;
+  hipLaunchKernelGGL(reorder_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, st[cur], st[nxt], vs, n);
+  cur = nxt;
+  hipLaunchKernelGGL(block_bounds_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, d_bend);
+  hipLaunchKernelGGL(chunk_flags_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, chunk,
+                     d_cflags);
+  exclusive_scan(d_cflags, d_cidx, n, d_part, nullptr, s);
+  hipLaunchKernelGGL(chunk_write_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_cflags, d_cidx,
+                     d_bend, d_block_scene, chunk, d_chunks, d_nchunks);
+}
+
+void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
+  const int g = num_cus * grid_mult;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timing) {
+    e0 = tevent();
+    e1 = tevent();
+    hipEventRecord(e0, s);
+  }
+  if (mode == 0) hipLaunchKernelGGL(transfer_kernel<0>, dim3(g), dim3(ENG_THREADS), 0, s, a);
+  else if (mode == 1) hipLaunchKernelGGL(transfer_kernel<1>, dim3(g), dim3(ENG_THREADS), 0, s, a);
+  else hipLaunchKernelGGL(transfer_kernel<2>, dim3(g), dim3(ENG_THREADS), 0, s, a);
+  if (timing) {
+    hipEventRecord(e1, s);
+    tpairs.push_back({mode, e0, e1});
+  }
+}
+
+void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums) {
+  const int g = num_cus * 2;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (timing) {
+    e0 = tevent();
+    e1 = tevent();
+    hipEventRecord(e0, s);
+  }
+  hipLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, a, with_sums);
+  if (timing) {
+    hipEventRecord(e1, s);
+    tpairs.push_back({3, e0, e1});
+  }
+}
+
+hipEvent_t Engine::tevent() {
+  if (tnext < tevents.size()) return tevents[tnext++];
+  hipEvent_t e;
+  hipEventCreate(&e);
+  tevents.push_back(e);
+  ++tnext;
+  return e;
+}
+
+// One segment of `len` substeps: bin, P2G, grid, (G2P2G, grid) x (len-1), G2P.
+void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps) {
+  enqueue_bin(s);
+  EngineArgs a = args(sums, max_steps);
+  const int ws = sums ? 1 : 0;
+  launch_transfer(0, s, a);
+  launch_grid(s, a, ws);
+  for (int k = 1; k < len; ++k) {
+    launch_transfer(1, s, a);
+    launch_grid(s, a, ws);
+  }
+  launch_transfer(2, s, a);
+}
+
+int Engine::run(int nsteps, double* sums) {
+  if (nsteps < 0) {
+    set_error("mpm_run: nsteps must be >= 0");
+    return MPM_ERR_ARG;
+  }
+  if (nsteps == 0) return MPM_OK;
+  HIPCHK(hipEventRecord(ev_in, user_stream));
+  HIPCHK(hipStreamWaitEvent(own_stream, ev_in, 0));
+  if (sums) {
+    hipLaunchKernelGGL(fill_f64_kernel, dim3(nblk((int64_t)nsteps * 8, 256)), dim3(256), 0, own_stream,
+                       sums, (int64_t)nsteps * 8, 0.0);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, own_stream, d_counters + 2, (int64_t)1, 0u);
+  }
+  int done = 0;
+  while (done < nsteps) {
+    const int len = (nsteps - done) < sort_every ? (nsteps - done) : sort_every;
+    const bool graphed = use_graph && !timing && n > 0;
+    if (graphed) {
+      const int parity = cur;
+      const GraphKey key{len, parity, (uintptr_t)sums, sums ? nsteps : 0};
+      auto it = graphs.find(key);
+      hipGraphExec_t exec = nullptr;
+      if (it == graphs.end()) {
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(own_stream, hipStreamCaptureModeThreadLocal));
+        enqueue_segment(len, own_stream, sums, nsteps);
+        HIPCHK(hipStreamEndCapture(own_stream, &graph));
+        HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(graph));
+        graphs[key] = exec;
+        cur = parity;  // capture toggled the host-side buffer index; replay does it below
+      } else {
+        exec = it->second;
+      }
+      HIPCHK(hipGraphLaunch(exec, own_stream));
+      cur = parity ^ 1;
+    } else {
+      enqueue_segment(len, own_stream, sums, nsteps);
+    }
+    HIPCHK(hipGetLastError());
+    done += len;
+  }
+  HIPCHK(hipEventRecord(ev_out, own_stream));
+  HIPCHK(hipStreamWaitEvent(user_stream, ev_out, 0));
+  return MPM_OK;
+}
+
+int Engine::take_counter(int which, int64_t* out) {
+  HIPCHK(hipStreamSynchronize(own_stream));
+  unsigned long long v = 0;
+  unsigned long long* p = which == 0 ? d_oob : d_fallback;
+  HIPCHK(hipMemcpy(&v, p, sizeof(v), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemset(p, 0, sizeof(v)));
+  *out = (int64_t)v;
+  return MPM_OK;
+}
+
+int Engine::particle_sums(double* out) {
+  double* d = nullptr;
+  int rc;
+  if ((rc = alloc((void**)&d, sizeof(double) * 8))) return rc;
+  HIPCHK(hipStreamSynchronize(own_stream));
+  HIPCHK(hipMemset(d, 0, sizeof(double) * 8));
+  if (n > 0)
+    hipLaunchKernelGGL(particle_sums_kernel, dim3(256), dim3(ENG_THREADS), 0, own_stream, st[cur], n, d);
+  HIPCHK(hipStreamSynchronize(own_stream));
+  HIPCHK(hipMemcpy(out, d, sizeof(double) * 7, hipMemcpyDeviceToHost));
+  allocs.pop_back();
+  HIPCHK(hipFree(d));
+  return MPM_OK;
+}
+
+int Engine::kernel_times(double* ms, int64_t* counts) {
+  HIPCHK(hipStreamSynchronize(own_stream));
+  for (int i = 0; i < 4; ++i) {
+    ms[i] = 0;
+    counts[i] = 0;
+  }
+  for (auto& tp : tpairs) {
+    float t = 0;
+    HIPCHK(hipEventElapsedTime(&t, tp.e0, tp.e1));
+    ms[tp.kind] += t;
+    counts[tp.kind] += 1;
+  }
+  tpairs.clear();
+  tnext = 0;
+  return MPM_OK;
+}
+
+}  // namespace mpm

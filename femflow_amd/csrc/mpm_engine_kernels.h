@@ -1,0 +1,91 @@
+// Device-side data structures and kernels of the particle-resident engine.
+//
+// Grid storage is block-sparse and block-major: every scene's (R+1)^3 node
+// lattice is cut into 4x4x4-node blocks (nb = ceil((R+1)/4) per axis), the
+// blocks of all scenes are concatenated, and a node lives at
+//   (block_base + (bi*nb + bj)*nb + bk) * 64 + (ii*16 + jj*4 + kk)
+// as one float4 (momentum or velocity xyz, mass).  Particles are binned by
+// (global block, cell in block) and processed in chunks of at most CHUNK
+// particles from one block.  A chunk's grid footprint is staged in an LDS
+// arena of 8x8x8 nodes whose origin is the block origin minus one node, so
+// particles that have drifted up to one cell out of their block (the grid
+// update clamps |v| to 0.9 dx/dt per axis) still land in it; anything further
+// takes the global-atomic path, so results never depend on binning.
+#pragma once
+#include "mpm_math.h"
+
+namespace mpm {
+
+constexpr int ENG_THREADS = 256;
+constexpr int ARENA = 8;  // nodes per axis
+constexpr int ARENA_NODES = ARENA * ARENA * ARENA;
+constexpr int MAX_SCENES = 4096;
+
+struct SceneDev {
+  int res, nb, block_base, boundary, bound, quirks;
+  float inv_dx, dx, dt, gravity, vol, v_allowed;
+};
+
+struct EngineState {
+  float* x;
+  float* v;
+  float* F;
+  float* C;
+  float* Jp;
+  float* mass;
+  float* mu0;
+  float* lam0;
+  uint32_t* attr;   // material id | scene << 8
+  uint32_t* order;  // caller index of this slot
+};
+
+struct EngineArgs {
+  EngineState s;
+  const SceneDev* scenes;
+  const MatParams* mats;
+  int nmat;
+  float4* acc;
+  float4* vel;
+  uint32_t* flags;
+  uint32_t* list;
+  uint32_t* counters;  // [0] active blocks, [1] finished workgroups, [2] step
+  const int4* chunks;  // start, count, global block, scene
+  const uint32_t* nchunks;
+  const int* block_scene;
+  unsigned long long* oob;
+  unsigned long long* fallback;
+  double* sums;
+  int max_steps;
+};
+
+__device__ __forceinline__ size_t node_addr(const SceneDev& sc, int i, int j, int k) {
+  const int blk = sc.block_base + ((i >> 2) * sc.nb + (j >> 2)) * sc.nb + (k >> 2);
+  return (size_t)blk * 64 + (size_t)(((i & 3) << 4) | ((j & 3) << 2) | (k & 3));
+}
+
+// Stencil validity and (base, fx) for one particle (three_d/p2g.py:50-53 with
+// the bounds test of utils.py:138-150).
+__device__ __forceinline__ bool locate3(const float* xp, const SceneDev& sc, int base[3],
+                                        float fx[3]) {
+  bool ok = true;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float s = xp[d] * sc.inv_dx;
+    const float t = s - 0.5f;
+    ok = ok && (t > -1.0f) && (t < (float)(sc.res - 1));
+    base[d] = ok ? (int)t : 0;
+    fx[d] = s - (float)base[d];
+  }
+  return ok;
+}
+
+__device__ __forceinline__ void mark_active(const EngineArgs& a, uint32_t g) {
+  if (__hip_atomic_load(a.flags + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+    if (atomicOr(a.flags + g, 1u) == 0u) {
+      const uint32_t idx = atomicAdd(a.counters, 1u);
+      a.list[idx] = g;
+    }
+  }
+}
+
+}  // namespace mpm

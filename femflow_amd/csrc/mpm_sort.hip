@@ -1,0 +1,282 @@
+// Cell binning: key generation, a stable LSD radix sort on 32-bit keys
+// (8-bit digits), a three-phase exclusive scan, and the chunk builder that
+// turns the sorted keys into per-block work items for the transfer kernels.
+//
+// Stability: inside a tile each wave owns a contiguous run of keys and walks
+// it in index order; ranks among equal digits come from a 64-lane match mask
+// built with eight ballots plus per-wave running counters, and waves are
+// ordered by a prefix over their counters.  Tiles are ordered by the digit-
+// major exclusive scan of the per-tile histograms.  Each pass is therefore a
+// stable counting sort, and the whole sort equals a stable sort by key.
+#include "mpm_sort.h"
+
+namespace mpm {
+
+constexpr int RS_THREADS = 256;
+constexpr int RS_WAVES = RS_THREADS / 64;
+constexpr int RS_ITEMS = 16;
+constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int SCAN_THREADS = 256;
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const unsigned lane = __lane_id();
+  return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
+}
+
+// Lanes (among `valid`) whose 8-bit digit equals this lane's.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
+  uint64_t m = valid;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    m &= bit ? bal : ~bal;
+  }
+  return m;
+}
+
+// Per-tile digit histogram, written digit-major: hist[d * ntiles + tile].
+__global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint32_t* __restrict__ keys,
+                                                         int64_t n, int shift,
+                                                         uint32_t* __restrict__ hist, int ntiles) {
+  __shared__ uint32_t h[256];
+  const int t = threadIdx.x;
+  h[t] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  const unsigned lane = __lane_id();
+#pragma unroll 4
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int64_t idx = base + (int64_t)r * RS_THREADS + t;
+    const bool valid = idx < n;
+    const uint32_t d = valid ? (keys[idx] >> shift) & 255u : 0u;
+    const uint64_t vm = __ballot(valid);
+    const uint64_t m = match_digit(d, vm);
+    if (valid && lane == 63u - (unsigned)__clzll(m)) atomicAdd(&h[d], (uint32_t)__popcll(m));
+  }
+  __syncthreads();
+  hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
+}
+
+__global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ offs, int ntiles,
+    int iota_values) {
+  __shared__ uint32_t cnt[RS_WAVES][256];
+  __shared__ uint32_t goff[256];
+  const int t = threadIdx.x;
+  const int w = t >> 6;
+  const unsigned lane = __lane_id();
+#pragma unroll
+  for (int i = 0; i < RS_WAVES; ++i) cnt[i][t] = 0;
+  goff[t] = offs[(int64_t)t * ntiles + blockIdx.x];
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * 64 * RS_ITEMS;
+  uint32_t k[RS_ITEMS];
+  uint32_t rank[RS_ITEMS];
+  const uint64_t lt = lanemask_lt();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int64_t idx = base + r * 64 + lane;
+    const bool valid = idx < n;
+    k[r] = valid ? kin[idx] : 0u;
+    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint64_t vm = __ballot(valid);
+    const uint64_t m = match_digit(d, vm);
+    uint32_t prev = 0;
+    if (valid) prev = cnt[w][d];
+    rank[r] = prev + (uint32_t)__popcll(m & lt);
+    if (valid && lane == 63u - (unsigned)__clzll(m)) cnt[w][d] = prev + (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  {
+    uint32_t run = 0;
+#pragma unroll
+    for (int i = 0; i < RS_WAVES; ++i) {
+      const uint32_t c = cnt[i][t];
+      cnt[i][t] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int64_t idx = base + r * 64 + lane;
+    if (idx < n) {
+      const uint32_t d = (k[r] >> shift) & 255u;
+      const uint32_t dst = goff[d] + cnt[w][d] + rank[r];
+      kout[dst] = k[r];
+      vout[dst] = iota_values ? (uint32_t)idx : vin[idx];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- scan ----
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* sh, uint32_t* total) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < SCAN_THREADS; off <<= 1) {
+    const uint32_t a = t >= off ? sh[t - off] : 0u;
+    __syncthreads();
+    sh[t] += a;
+    __syncthreads();
+  }
+  const uint32_t incl = sh[t];
+  *total = sh[SCAN_THREADS - 1];
+  __syncthreads();
+  return incl - v;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void scan_reduce(const uint32_t* __restrict__ in,
+                                                            int64_t n, uint32_t* __restrict__ part) {
+  __shared__ uint32_t sh[SCAN_THREADS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) s += (base + i < n) ? in[base + i] : 0u;
+  uint32_t tot;
+  block_exclusive_scan(s, sh, &tot);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// Single workgroup: exclusive scan of the per-tile partial sums in place.
+__global__ __launch_bounds__(SCAN_THREADS) void scan_partials(uint32_t* __restrict__ part, int64_t np,
+                                                              uint32_t* __restrict__ grand_total) {
+  __shared__ uint32_t sh[SCAN_THREADS];
+  uint32_t carry = 0;
+  for (int64_t b = 0; b < np; b += SCAN_THREADS) {
+    const int64_t i = b + threadIdx.x;
+    const uint32_t v = i < np ? part[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_exclusive_scan(v, sh, &tot);
+    if (i < np) part[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0 && grand_total) *grand_total = carry;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void scan_downsweep(const uint32_t* __restrict__ in,
+                                                               int64_t n, const uint32_t* __restrict__ part,
+                                                               uint32_t* __restrict__ out) {
+  __shared__ uint32_t sh[SCAN_THREADS];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+  uint32_t v[SCAN_ITEMS];
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    v[i] = (base + i < n) ? in[base + i] : 0u;
+    s += v[i];
+  }
+  uint32_t tot;
+  uint32_t run = block_exclusive_scan(s, sh, &tot) + part[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < SCAN_ITEMS; ++i) {
+    if (base + i < n) out[base + i] = run;
+    run += v[i];
+  }
+}
+
+// ---------------------------------------------------------------- keys ----
+// base = trunc(fl32(fl32(x*inv_dx) - 0.5)), clamped to [0, res] (the same
+// rounding the reference's (x*inv_dx - 0.5).astype(int64) sees on FP32 x).
+__device__ __forceinline__ int cell_coord(float x, float inv_dx, int res) {
+  const float s = __fmul_rn(x, inv_dx);
+  const float t = __fsub_rn(s, 0.5f);
+  int b;
+  if (!(t > -1.0f)) b = 0;            // also NaN
+  else if (!(t < (float)res)) b = res;
+  else b = (int)t;                    // truncation toward zero
+  return b < 0 ? 0 : (b > res ? res : b);
+}
+
+__global__ void bin_keys_kernel(const float* __restrict__ x, int64_t n, int dim, int res,
+                                float inv_dx, int layout, uint32_t* __restrict__ keys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int b[3] = {0, 0, 0};
+  for (int d = 0; d < dim; ++d) b[d] = cell_coord(x[i * dim + d], inv_dx, res);
+  const uint32_t r1 = (uint32_t)res + 1u;
+  uint32_t key;
+  if (layout == MPM_KEY_ROWMAJOR) {
+    key = (uint32_t)b[0];
+    for (int d = 1; d < dim; ++d) key = key * r1 + (uint32_t)b[d];
+  } else {
+    const uint32_t nb = (r1 + 3u) / 4u;
+    uint32_t bid = (uint32_t)(b[0] >> 2), cid = (uint32_t)(b[0] & 3);
+    for (int d = 1; d < dim; ++d) {
+      bid = bid * nb + (uint32_t)(b[d] >> 2);
+      cid = cid * 4u + (uint32_t)(b[d] & 3);
+    }
+    key = bid * (dim == 3 ? 64u : 16u) + cid;
+  }
+  keys[i] = key;
+}
+
+// ------------------------------------------------------------ host side ----
+static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+size_t radix_workspace_bytes(int64_t n) {
+  const int64_t ntiles = cdiv(n > 0 ? n : 1, RS_TILE);
+  const int64_t hist = 256 * ntiles;
+  const int64_t parts = cdiv(hist > n ? hist : n, SCAN_TILE) + 1;
+  // keys/vals ping-pong (4 arrays of n) + hist + offs + partials + total
+  return sizeof(uint32_t) * (4 * (size_t)n + 2 * (size_t)(hist > n ? hist : n) + parts + 64);
+}
+
+void exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* part,
+                    uint32_t* total, hipStream_t st) {
+  if (n <= 0) return;
+  const int64_t nb = cdiv(n, SCAN_TILE);
+  hipLaunchKernelGGL(scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, in, n, part);
+  hipLaunchKernelGGL(scan_partials, dim3(1), dim3(SCAN_THREADS), 0, st, part, nb, total);
+  hipLaunchKernelGGL(scan_downsweep, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, in, n, part, out);
+}
+
+int radix_passes(uint32_t max_key) {
+  int bits = 0;
+  while (bits < 32 && (max_key >> bits) != 0u) ++bits;
+  return bits == 0 ? 1 : (bits + 7) / 8;
+}
+
+// Stable sort of keys[0..n) carrying values; result in (*kout, *vout), which
+// point into either the input arrays or the workspace.  iota_values: the
+// input values are 0..n-1 (not read).
+void radix_sort(uint32_t* keys, uint32_t* vals, int64_t n, uint32_t max_key, void* ws,
+                bool iota_values, uint32_t** kout, uint32_t** vout, hipStream_t st) {
+  const int64_t ntiles = cdiv(n > 0 ? n : 1, RS_TILE);
+  const int64_t hist_n = 256 * ntiles;
+  const int64_t big = hist_n > n ? hist_n : n;
+  uint32_t* w = (uint32_t*)ws;
+  uint32_t* k2 = w;
+  uint32_t* v2 = w + n;
+  uint32_t* hist = w + 2 * n;
+  uint32_t* offs = hist + big;
+  uint32_t* part = offs + big;
+  uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
+  const int passes = radix_passes(max_key);
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, n, shift,
+                       hist, (int)ntiles);
+    exclusive_scan(hist, offs, hist_n, part, nullptr, st);
+    hipLaunchKernelGGL(rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, va, kb, vb,
+                       n, shift, offs, (int)ntiles, (iota_values && p == 0) ? 1 : 0);
+    uint32_t* t;
+    t = ka; ka = kb; kb = t;
+    t = va; va = vb; vb = t;
+  }
+  *kout = ka;
+  *vout = va;
+}
+
+void bin_keys(const float* x, int64_t n, int dim, int res, float inv_dx, int layout, uint32_t* keys,
+              hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(bin_keys_kernel, dim3((unsigned)cdiv(n, 256)), dim3(256), 0, st, x, n, dim, res,
+                     inv_dx, layout, keys);
+}
+
+}  // namespace mpm

@@ -1,0 +1,196 @@
+"""Particle-resident MLS-MPM engine (many scenes, block-sparse grids, FP32).
+
+Wraps the ``mpm_ctx`` half of the C ABI.  One ``Engine`` owns a binned copy of
+the particle state on one GPU and advances every scene by whole substeps of
+the reference step (femflow/solvers/mpm/mls_mpm.py:40-79): P2G, grid update,
+G2P.  Between rebinning points the G2P of one substep and the P2G of the next
+run fused in one kernel; the sequence is captured once as a hipGraph and
+replayed.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass
+from typing import Optional, Sequence
+
+import torch
+
+from . import _lib
+
+KINDS = {"elastic": _lib.MPM_ELASTIC, "snow": _lib.MPM_SNOW, "water": _lib.MPM_WATER}
+
+
+@dataclass
+class SceneSpec:
+    """One scene: a (res+1)^3 grid on [0,1]^3 with its own time step."""
+
+    res: int
+    dt: float
+    gravity: float = -9.8
+    volume: float = 1.0
+    boundary: str = "reference"  # "reference" (three_d/grid_op.py:37-47) or "sticky"
+    bound: int = 3
+    quirks: str = "reference"     # "reference" (utils.py:129 broadcast) or "textbook"
+
+
+@dataclass
+class MaterialSpec:
+    kind: str = "elastic"  # elastic | snow | water
+    hardening: float = 1.0
+    theta_c: float = 2.5e-2
+    theta_s: float = 7.5e-3
+
+
+class Engine:
+    def __init__(self, scenes: Sequence[SceneSpec], materials: Sequence[MaterialSpec], n_max: int,
+                 device=None):
+        _lib.require_gpu()
+        self._L = _lib.load()
+        self.device = torch.device(device) if device is not None else torch.device(
+            "cuda", torch.cuda.current_device())
+        self.scenes = list(scenes)
+        self.materials = list(materials)
+        sc = (_lib.Scene * len(self.scenes))()
+        for i, s in enumerate(self.scenes):
+            sc[i] = _lib.Scene(int(s.res),
+                               _lib.MPM_BC_REFERENCE if s.boundary == "reference" else _lib.MPM_BC_STICKY,
+                               int(s.bound),
+                               _lib.MPM_QUIRKS_REFERENCE if s.quirks == "reference" else _lib.MPM_QUIRKS_TEXTBOOK,
+                               float(s.dt), float(s.gravity), float(s.volume))
+        mt = (_lib.Material * len(self.materials))()
+        for i, m in enumerate(self.materials):
+            mt[i] = _lib.Material(KINDS[m.kind] if isinstance(m.kind, str) else int(m.kind),
+                                  float(m.hardening), float(m.theta_c), float(m.theta_s))
+        h = ct.c_void_p()
+        with torch.cuda.device(self.device):
+            _lib.check(self._L.mpm_ctx_create(ct.byref(h), int(n_max), len(self.scenes), sc,
+                                              len(self.materials), mt, 0), "mpm_ctx_create")
+        self._h = h
+        self.n_max = int(n_max)
+        self.n = 0
+        self._material = None
+        self._scene = None
+        self.tune()
+
+    # ------------------------------------------------------------------
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _sync_stream(self):
+        _lib.check(self._L.mpm_ctx_set_stream(self._h, self._stream()), "set_stream")
+
+    def tune(self, chunk: int = 512, sort_every: int = 8, grid_mult: int = 4, use_graph: bool = True):
+        self.config = dict(chunk=chunk, sort_every=sort_every, grid_mult=grid_mult, use_graph=use_graph)
+        _lib.check(self._L.mpm_ctx_tune(self._h, chunk, sort_every, grid_mult, 1 if use_graph else 0),
+                   "tune")
+
+    def _particles(self, x, v, F, C, Jp, mass=None, mu0=None, lam0=None, material=None):
+        def ptr(t):
+            return None if t is None else t.data_ptr()
+
+        return _lib.Particles(int(x.shape[0]), _lib.MPM_F32, 0, ptr(x), ptr(v), ptr(F), ptr(C), ptr(Jp),
+                              ptr(mass), ptr(mu0), ptr(lam0), ptr(material))
+
+    @staticmethod
+    def _check_tensor(t, name, dtype=torch.float32):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous()):
+            raise ValueError(f"{name} must be a contiguous CUDA {dtype} tensor")
+
+    def upload(self, x, v, F, C, Jp, mass, mu0, lam0, material=None, scene=None):
+        """Load FP32 CUDA tensors in the reference layouts (x, v: (N,3); F, C:
+        (N,3,3); Jp: (N,) or (N,1); mass/mu0/lam0: (N,)); material/scene: int32 (N,)."""
+        for t, nm in ((x, "x"), (v, "v"), (F, "F"), (C, "C"), (Jp, "Jp"), (mass, "mass"), (mu0, "mu0"),
+                      (lam0, "lam0")):
+            self._check_tensor(t, nm)
+        for t, nm, hi in ((material, "material", len(self.materials)), (scene, "scene", len(self.scenes))):
+            if t is not None:
+                self._check_tensor(t, nm, torch.int32)
+                if t.numel() and (int(t.min()) < 0 or int(t.max()) >= hi):
+                    raise ValueError(f"{nm} ids must lie in [0, {hi})")
+        if int(x.shape[0]) > self.n_max:
+            raise ValueError(f"{x.shape[0]} particles exceed n_max={self.n_max}")
+        self._sync_stream()
+        p = self._particles(x, v, F, C, Jp, mass, mu0, lam0, material)
+        _lib.check(self._L.mpm_state_upload(self._h, ct.byref(p),
+                                            None if scene is None else scene.data_ptr()), "upload")
+        self.n = int(x.shape[0])
+
+    def download(self, x, v, F, C, Jp):
+        for t, nm in ((x, "x"), (v, "v"), (F, "F"), (C, "C"), (Jp, "Jp")):
+            self._check_tensor(t, nm)
+        self._sync_stream()
+        p = self._particles(x, v, F, C, Jp)
+        _lib.check(self._L.mpm_state_download(self._h, ct.byref(p)), "download")
+
+    def run(self, nsteps: int, sums: Optional[torch.Tensor] = None, check_oob: bool = True):
+        """Advance every scene by ``nsteps`` substeps.  ``sums``: optional FP64
+        CUDA tensor (nsteps, 8) receiving per-substep grid conservation sums
+        [mass, momentum after P2G (3), momentum after the grid update (3), 0]."""
+        self._sync_stream()
+        if sums is not None:
+            self._check_tensor(sums, "sums", torch.float64)
+            if sums.numel() < nsteps * 8:
+                raise ValueError("sums needs nsteps*8 entries")
+        _lib.check(self._L.mpm_run(self._h, int(nsteps), None if sums is None else sums.data_ptr()),
+                   "run")
+        if check_oob:
+            n = self.oob_count()
+            if n:
+                raise RuntimeError(f"{n} particle stencil(s) left the grid")
+
+    def oob_count(self) -> int:
+        out = ct.c_int64()
+        _lib.check(self._L.mpm_ctx_oob_count(self._h, ct.byref(out)), "oob_count")
+        return int(out.value)
+
+    def fallback_count(self) -> int:
+        out = ct.c_int64()
+        _lib.check(self._L.mpm_ctx_fallback_count(self._h, ct.byref(out)), "fallback_count")
+        return int(out.value)
+
+    def particle_sums(self):
+        out = (ct.c_double * 7)()
+        _lib.check(self._L.mpm_particle_sums(self._h, out), "particle_sums")
+        return list(out)
+
+    def kernel_timing(self, enable: bool = True):
+        _lib.check(self._L.mpm_ctx_kernel_timing(self._h, 1 if enable else 0), "kernel_timing")
+
+    def kernel_times(self):
+        """{name: (total_ms, launches)} since the previous call (timing mode)."""
+        ms = (ct.c_double * 4)()
+        cnt = (ct.c_int64 * 4)()
+        _lib.check(self._L.mpm_ctx_kernel_times(self._h, ms, cnt), "kernel_times")
+        names = ("p2g", "g2p2g", "g2p", "grid_update")
+        return {nm: (ms[i], int(cnt[i])) for i, nm in enumerate(names)}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.mpm_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        import sys
+
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bin_particles(x: torch.Tensor, res: int, inv_dx: float, layout: str = "blocked"):
+    """Cell keys and their stable ordering for FP32 CUDA positions (N, d)."""
+    _lib.require_gpu()
+    L = _lib.load()
+    if not (x.is_cuda and x.dtype == torch.float32 and x.is_contiguous() and x.dim() == 2):
+        raise ValueError("x must be a contiguous CUDA float32 (N, d) tensor")
+    n, dim = x.shape
+    keys = torch.empty(n, dtype=torch.int32, device=x.device)
+    perm = torch.empty(n, dtype=torch.int32, device=x.device)
+    torch.cuda.current_stream(x.device).synchronize()
+    lay = _lib.MPM_KEY_BLOCKED if layout == "blocked" else _lib.MPM_KEY_ROWMAJOR
+    _lib.check(L.mpm_bin(None, int(dim), int(res), float(inv_dx), lay, x.data_ptr(), int(n),
+                         keys.data_ptr(), perm.data_ptr()), "mpm_bin")
+    return keys, perm

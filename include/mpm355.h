@@ -1,0 +1,187 @@
+/*
+ * mpm355 -- C ABI of the MI355X (gfx950) explicit MLS-MPM substep.
+ *
+ * Every entry point takes plain pointers and sizes; pointers named "device"
+ * are HIP device addresses (for example torch.Tensor.data_ptr() of a CUDA
+ * tensor on ROCm) and stream arguments are hipStream_t values (NULL = the
+ * legacy default stream).  Each function returns an mpm_status; the message
+ * of the most recent failure on the calling thread is mpm_last_error_string().
+ *
+ * The reference interface each entry point replaces is cited with paths
+ * relative to the FEMFlow source tree (jparr721/FEMFlow).
+ */
+#ifndef MPM355_H
+#define MPM355_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define MPM_OK 0
+#define MPM_ERR_OOB 1       /* a particle's stencil left [0, res]: utils.py:138-150 raise */
+#define MPM_ERR_NONFINITE 2 /* NaN/Inf found by a conservation check            */
+#define MPM_ERR_ARG 3       /* bad argument (null pointer, size, dtype ...)       */
+#define MPM_ERR_HIP 4       /* a HIP runtime call failed                          */
+
+/* ---- enums ----------------------------------------------------------- */
+#define MPM_F32 0
+#define MPM_F64 1
+
+#define MPM_ELASTIC 0 /* fixed-corotated, (mu, lam) = (mu0 h, lam0 h): utils.py:7-24      */
+#define MPM_SNOW 1    /* e = exp(h (1 - Jp)) hardening + singular value clamp: utils.py:27-49, three_d/g2p.py:48-58 */
+#define MPM_WATER 2   /* weakly compressible: PF = K (J-1) J I, F <- cbrt(J) I (build-only) */
+
+#define MPM_QUIRKS_REFERENCE 0 /* lambda (J-1) J added to all d*d entries (utils.py:129) */
+#define MPM_QUIRKS_TEXTBOOK 1  /* lambda (J-1) J * I                                    */
+
+#define MPM_BC_REFERENCE 0 /* 3D: three_d/grid_op.py:37-47; 2D: two_d/grid_op.py:19-24 */
+#define MPM_BC_STICKY 1    /* every component zero within `bound` nodes of a wall       */
+
+#define MPM_KEY_BLOCKED 0  /* (4^d-cell block id, cell within block), both row-major */
+#define MPM_KEY_ROWMAJOR 1 /* the grid's own [i, j(, k)] row-major node order          */
+
+/* ---- plain data ------------------------------------------------------ */
+typedef struct {
+  int32_t kind;     /* MPM_ELASTIC / MPM_SNOW / MPM_WATER */
+  float hardening;  /* h: the reference's `hardening` argument (p2g.py:16) */
+  float theta_c;    /* snow: singular values clamped to [1 - theta_c, 1 + theta_s] */
+  float theta_s;
+} mpm_material;
+
+/* Step parameters: the scalar arguments of solve_mls_mpm_3d (mls_mpm.py:40-53)
+ * and of the phase functions (three_d/p2g.py:15-29, grid_op.py:6-13). */
+typedef struct {
+  int32_t dim;      /* 2 or 3 */
+  int32_t res;      /* grid resolution R: (R+1)^dim nodes */
+  double inv_dx, dx, dt, volume, gravity;
+  int32_t boundary; /* MPM_BC_* */
+  int32_t bound;    /* sticky wall thickness in nodes (MPM_BC_STICKY only) */
+  int32_t quirks;   /* MPM_QUIRKS_* */
+  int32_t check_oob;/* 1: count particles whose stencil leaves [0, R] (3D reference) */
+} mpm_params;
+
+/* Particle arrays in the reference's layouts (simulation.py:99-108):
+ * x, v: (n, dim); F, C: (n, dim, dim) row-major; Jp: (n) [the reference's
+ * (n, 1)]; mass, mu0, lambda0: (n), or one value each when mat_scalar != 0
+ * (the 2D signature, two_d/p2g.py:12-29); material: (n) ids into a material
+ * table, or NULL for id 0.  All device pointers of dtype `dtype`. */
+typedef struct {
+  int64_t n;
+  int32_t dtype;      /* MPM_F32 / MPM_F64 */
+  int32_t mat_scalar;
+  void* x;
+  void* v;
+  void* F;
+  void* C;
+  void* Jp;
+  const void* mass;
+  const void* mu0;
+  const void* lambda0;
+  const int32_t* material;
+} mpm_particles;
+
+/* One scene of the batched engine (a grid with its own resolution, time
+ * step, gravity and walls). */
+typedef struct {
+  int32_t res;
+  int32_t boundary;
+  int32_t bound;
+  int32_t quirks;
+  double dt, gravity, volume;
+} mpm_scene;
+
+/* ---- library ----------------------------------------------------------- */
+const char* mpm_last_error_string(void);
+int32_t mpm_version(void);
+/* Number of visible HIP devices (0 when none); never fails for lack of a GPU. */
+int32_t mpm_device_count(void);
+
+/* ---- phase entry points on caller grids (dense reference layouts) ------
+ * grid_v: (R+1)^dim x dim, grid_m: (R+1)^dim, row-major [i, j(, k)], dtype of
+ * the particles.  oob_count (device int64, may be NULL) is incremented by the
+ * number of particles whose stencil leaves the grid (those are skipped). */
+
+/* Replaces three_d.p2g (three_d/p2g.py:14-80) and two_d.p2g
+ * (two_d/p2g.py:11-76).  kind_override >= 0 forces one material kind for all
+ * particles with hardening `hardening` (the reference's model string:
+ * "neo_hookean" -> MPM_ELASTIC, anything else -> MPM_SNOW); -1 uses the
+ * per-particle material ids with `materials` (host array of nmat). */
+int32_t mpm_p2g(void* stream, const mpm_params* prm, const mpm_particles* p,
+                int32_t kind_override, double hardening, const mpm_material* materials,
+                int32_t nmat, void* grid_v, void* grid_m, int64_t* oob_count);
+
+/* Replaces three_d.grid_op (three_d/grid_op.py:5-47) and two_d.grid_op
+ * (two_d/grid_op.py:5-24). In place on grid_v. */
+int32_t mpm_grid_op(void* stream, const mpm_params* prm, int32_t dtype, void* grid_v,
+                    const void* grid_m);
+
+/* Replaces three_d.g2p (three_d/g2p.py:9-59) and two_d.g2p (two_d/g2p.py:5-47).
+ * kind_override >= 0: MPM_SNOW clamps with theta_c/theta_s for every
+ * particle (model == "snow"), MPM_ELASTIC never clamps. */
+int32_t mpm_g2p(void* stream, const mpm_params* prm, mpm_particles* p, int32_t kind_override,
+                double theta_c, double theta_s, const mpm_material* materials, int32_t nmat,
+                const void* grid_v, int64_t* oob_count);
+
+/* ---- engine: particle-resident rollout on block-sparse grids -----------
+ * No reference counterpart as a single function: it runs the loop of
+ * simulation.py:125-139 (solve_mls_mpm_3d per substep, mls_mpm.py:40-79)
+ * for many substeps and many scenes without leaving the device.  FP32, 3D. */
+typedef struct mpm_ctx mpm_ctx;
+
+/* flags: reserved, pass 0.  res of every scene <= 1024. */
+int32_t mpm_ctx_create(mpm_ctx** out, int64_t n_max, int32_t nscenes, const mpm_scene* scenes,
+                       int32_t nmat, const mpm_material* materials, int32_t flags);
+int32_t mpm_ctx_destroy(mpm_ctx* ctx);
+int32_t mpm_ctx_set_stream(mpm_ctx* ctx, void* stream);
+/* Tunables: chunk = particles per work item (256, 512 or 1024); sort_every =
+ * substeps between rebinning (>= 1); grid_mult = persistent workgroups per CU;
+ * use_graph = capture and replay the substep sequence as a hipGraph. */
+int32_t mpm_ctx_tune(mpm_ctx* ctx, int32_t chunk, int32_t sort_every, int32_t grid_mult,
+                     int32_t use_graph);
+
+/* Per-kernel timing with HIP events on the engine's stream (eager launches
+ * only; graphs are not used while enabled).  mpm_ctx_kernel_times returns
+ * total milliseconds and launch counts for [P2G, fused G2P2G, G2P, grid
+ * update] since the previous call. */
+int32_t mpm_ctx_kernel_timing(mpm_ctx* ctx, int32_t enable);
+int32_t mpm_ctx_kernel_times(mpm_ctx* ctx, double* ms, int64_t* counts);
+
+/* Load particles (device FP32 arrays, reference layouts, material ids
+ * required unless nmat == 1) with an optional per-particle scene id (device
+ * int32, NULL = scene 0).  The engine keeps its own binned copy. */
+int32_t mpm_state_upload(mpm_ctx* ctx, const mpm_particles* p, const int32_t* scene_ids);
+/* Write the engine's state back to caller arrays in the caller's order
+ * (x, v, F, C, Jp). */
+int32_t mpm_state_download(mpm_ctx* ctx, mpm_particles* p);
+/* Advance every scene by nsteps substeps.  If sums != NULL (device double,
+ * nsteps * 8 entries) each substep writes FP64 [grid mass, grid momentum x3
+ * after P2G, grid momentum x3 after the update, particle-count] (see docs). */
+int32_t mpm_run(mpm_ctx* ctx, int32_t nsteps, double* sums);
+/* Convenience for the reference's per-call pattern: upload, run, download. */
+int32_t mpm_substep(mpm_ctx* ctx, mpm_particles* p, const int32_t* scene_ids, int32_t nsteps);
+/* Particles whose stencil left their grid since the context was created or
+ * this counter was last read (reading resets it).  Synchronises the stream. */
+int32_t mpm_ctx_oob_count(mpm_ctx* ctx, int64_t* out);
+/* FP32 polar/SVD evaluations that fell back to FP64 (reading resets it). */
+int32_t mpm_ctx_fallback_count(mpm_ctx* ctx, int64_t* out);
+/* FP64 sums over the engine's particles: out[0] = sum m, out[1..3] = sum m v,
+ * out[4..6] = sum m x (host double[7]). Synchronises the stream. */
+int32_t mpm_particle_sums(mpm_ctx* ctx, double* out);
+
+/* ---- binning: cell keys and the stable radix sort ----------------------
+ * keys[i] = cell key of trunc(fl32(fl32(x*inv_dx) - 0.5)) clamped to [0, res];
+ * perm = the stable ordering of keys (equal to numpy.argsort(keys,
+ * kind="stable")).  x: device float (n, dim).  keys, perm: device uint32 (n).
+ * No reference counterpart (the reference walks particles in list order,
+ * three_d/p2g.py:49); the key derives from its base, three_d/p2g.py:50. */
+int32_t mpm_bin(mpm_ctx* ctx, int32_t dim, int32_t res, double inv_dx, int32_t layout,
+                const float* x, int64_t n, uint32_t* keys, uint32_t* perm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MPM355_H */
