@@ -3,7 +3,7 @@
 
 #include <string>
 
-#include "../../include/mpm355.h"
+#include "mpm355.h"
 #include "mpm_engine.h"
 #include "mpm_phase.h"
 #include "mpm_sort.h"

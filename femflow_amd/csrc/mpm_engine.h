@@ -7,7 +7,7 @@
 #include <tuple>
 #include <vector>
 
-#include "../../include/mpm355.h"
+#include "mpm355.h"
 #include "mpm_engine_kernels.h"
 
 namespace mpm {
@@ -58,6 +58,7 @@ struct Engine {
   int4* d_chunks = nullptr;
   uint32_t *d_nchunks = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_cflags = nullptr,
            *d_cidx = nullptr, *d_part = nullptr;
+  uint32_t* d_sorted_keys = nullptr;
   void* d_ws = nullptr;
   size_t ws_bytes = 0;
 

@@ -82,14 +82,18 @@ __device__ __forceinline__ int advance(const SceneDev& sc, const MatParams& mp, 
     float U[9], s[3], V[9];
     fb = svd3_checked<float>(Fn, U, s, V);
     const float oldJ = det3(Fn);
+    const float lo = 1.f - mp.theta_c, hi = 1.f + mp.theta_s;
+    // rebuild only when a singular value is clamped (U diag(s) V^T is F otherwise)
+    if (s[0] < lo || s[0] > hi || s[1] < lo || s[1] > hi || s[2] < lo || s[2] > hi) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) s[c] = tmin(tmax(s[c], 1.f - mp.theta_c), 1.f + mp.theta_s);
+      for (int c = 0; c < 3; ++c) s[c] = tmin(tmax(s[c], lo), hi);
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int j = 0; j < 3; ++j)
-        Fn[3 * i + j] = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
-                        U[3 * i + 2] * s[2] * V[3 * j + 2];
+        for (int j = 0; j < 3; ++j)
+          Fn[3 * i + j] = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
+                          U[3 * i + 2] * s[2] * V[3 * j + 2];
+    }
     const float det = det3(Fn) + 1e-10f;
     Jp = tmin(tmax(Jp * oldJ / det, 0.6f), 20.f);
   } else if (mp.kind == MPM_WATER) {
@@ -196,13 +200,76 @@ __device__ __forceinline__ void store9(float* p, int64_t i, const float o[9]) {
 }
 
 // --------------------------------------------------------- transfer kernel ----
-// MODE 0: P2G only (after binning).  MODE 1: fused G2P + P2G.  MODE 2: G2P only.
+// One work item = one chunk of at most CHUNK binned particles of one 4x4x4-cell
+// block.  MODE 0: P2G only (right after binning).  MODE 1: fused G2P of this
+// substep + P2G of the next.  MODE 2: G2P only.
+//
+// P2G accumulates in an 8x8x8-node LDS arena in 32-bit fixed point: each
+// thread first computes its particles' P2G terms; the chunk sums, per
+// channel, every particle's bound on its largest single-node contribution
+// (which bounds any node's total from this chunk), scales so that sum maps to
+// 2^30, rounds each contribution to int32 and adds it with 32-bit integer LDS
+// atomics (float and 64-bit LDS atomics are an order of magnitude slower on
+// this hardware).  The flush converts each node back to FP32 for the global
+// atomics.  Integer sums are associative, so the in-chunk accumulation is
+// order independent.  Particles that left the arena (drift since binning)
+// take the global-atomic path.
 // Persistent: each XCD (workgroup id mod 8) walks a contiguous eighth of the
 // chunk list, so neighbouring chunks share that XCD's L2.
-template <int MODE>
+struct ChunkScale {
+  float s[4];     // float -> fixed point
+  double inv[4];  // fixed point -> float
+};
+
+__device__ __forceinline__ float warp_sum(float v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Largest |w_o (q0 + Adx o)| and |w_o m| over the 27 nodes, bounded by the
+// largest weight product times the largest |q0 + Adx o| (o in {0,1,2}^3).
+__device__ __forceinline__ void contribution_bound(const float w[3][3], const P2GTerms& t, float b[4]) {
+  float wm = 1.f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    wm *= fmaxf(fabsf(w[d][0]), fmaxf(fabsf(w[d][1]), fabsf(w[d][2])));
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    b[d] = wm * (fabsf(t.q0[d]) + 2.f * (fabsf(t.Adx[3 * d]) + fabsf(t.Adx[3 * d + 1]) + fabsf(t.Adx[3 * d + 2])));
+  b[3] = wm * fabsf(t.m);
+}
+
+__device__ __forceinline__ void scatter_fixed(uint32_t (*acc)[ARENA_NODES], int abase,
+                                              const float w[3][3], const P2GTerms& t, const float sc[4]) {
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const float wij = w[0][i] * w[1][j];
+      float qij[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) qij[d] = t.q0[d] + t.Adx[3 * d] * (float)i + t.Adx[3 * d + 1] * (float)j;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const float wt = wij * w[2][k];
+        const int off = abase + i * 64 + j * 8 + k;
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+          const float val = d < 3 ? wt * (qij[d] + t.Adx[3 * d + 2] * (float)k) : wt * t.m;
+          const int q = __float2int_rn(val * sc[d]);
+          atomicAdd(&acc[d][off], (uint32_t)q);
+        }
+      }
+    }
+}
+
+template <int MODE, int P>
 __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   __shared__ float4 vel_s[MODE == 0 ? 1 : ARENA_NODES];
-  __shared__ float acc_s[4][MODE == 2 ? 1 : ARENA_NODES];
+  __shared__ uint32_t acc_s[4][MODE == 2 ? 1 : ARENA_NODES];
+  __shared__ float red_s[ENG_THREADS / 64][4];
+  __shared__ ChunkScale cs;
   __shared__ uint32_t sub_active[27];
   __shared__ uint32_t wg_oob, wg_fb;
   const int t = threadIdx.x;
@@ -231,50 +298,60 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         vel_s[e] = in ? a.vel[node_addr(sc, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
       if constexpr (MODE != 2) {
-        acc_s[0][e] = 0.f;
-        acc_s[1][e] = 0.f;
-        acc_s[2][e] = 0.f;
-        acc_s[3][e] = 0.f;
+        acc_s[0][e] = 0u;
+        acc_s[1][e] = 0u;
+        acc_s[2][e] = 0u;
+        acc_s[3][e] = 0u;
       }
     }
     if (t < 27) sub_active[t] = 0;
     __syncthreads();
-    // lanes of one wave take particles S apart so that lanes rarely share a
-    // cell (and hence the same LDS addresses) in one atomic instruction
+    // Particle q of the chunk for (thread, r).  P2G modes space the lanes of
+    // a wave S = ceil(count/64) particles apart so that lanes rarely share a
+    // cell (the same LDS addresses) in one atomic; G2P-only is coalesced.
     const int S = (ch.y + 63) >> 6;
-    const int P = (S + 3) >> 2;
+    const int PP = (S + 3) >> 2;  // <= P
+    P2GTerms tm[P];
+    int base[P][3];
+    float fx[P][3];
+    bool act[P];
+    float bsum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
     for (int r = 0; r < P; ++r) {
-      const int jj = wv * P + r;
-      const int q = lane * S + jj;
-      if (jj >= S || q >= ch.y) continue;
+      act[r] = false;
+      int q;
+      if constexpr (MODE == 2) {
+        q = r * ENG_THREADS + t;
+        if (q >= ch.y) continue;
+      } else {
+        const int jj = wv * PP + r;
+        q = lane * S + jj;
+        if (r >= PP || jj >= S || q >= ch.y) continue;
+      }
       const int64_t p = (int64_t)ch.x + q;
-      float x[3], F[9], v[3], C[9], Jp;
+      float x[3], F[9], v[3], C[9], Jp, w[3][3];
       load3(a.s.x, p, x);
       load9(a.s.F, p, F);
       Jp = a.s.Jp[p];
-      const uint32_t attr = a.s.attr[p];
-      int mid = (int)(attr & 255u);
+      int mid = (int)(a.s.attr[p] & 255u);
       mid = mid < a.nmat ? mid : a.nmat - 1;
       const MatParams mp = a.mats[mid];
-      int base[3];
-      float fx[3], w[3][3];
       if constexpr (MODE == 0) {
         load3(a.s.v, p, v);
         load9(a.s.C, p, C);
       } else {
-        if (!locate3(x, sc, base, fx)) {
+        if (!locate3(x, sc, base[r], fx[r])) {
           atomicAdd(&wg_oob, 1u);
           continue;
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
-        const int a0 = base[0] - O0, a1 = base[1] - O1, a2 = base[2] - O2;
+        for (int d = 0; d < 3; ++d) bspline(fx[r][d], w[d]);
+        const int a0 = base[r][0] - O0, a1 = base[r][1] - O1, a2 = base[r][2] - O2;
         if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5)
-          gather<true>(vel_s, (a0 * 8 + a1) * 8 + a2, a.vel, sc, base, w, fx, v, C);
+          gather<true>(vel_s, (a0 * 8 + a1) * 8 + a2, a.vel, sc, base[r], w, fx[r], v, C);
         else
-          gather<false>(vel_s, 0, a.vel, sc, base, w, fx, v, C);
-        const int fb = advance(sc, mp, x, v, C, F, Jp);
-        if (fb) atomicAdd(&wg_fb, 1u);
+          gather<false>(vel_s, 0, a.vel, sc, base[r], w, fx[r], v, C);
+        if (advance(sc, mp, x, v, C, F, Jp)) atomicAdd(&wg_fb, 1u);
         store3(a.s.x, p, x);
         store3(a.s.v, p, v);
         store9(a.s.C, p, C);
@@ -282,34 +359,75 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         a.s.Jp[p] = Jp;
       }
       if constexpr (MODE != 2) {
-        if (!locate3(x, sc, base, fx)) {
+        if (!locate3(x, sc, base[r], fx[r])) {
           atomicAdd(&wg_oob, 1u);
           continue;
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
-        P2GTerms tm;
-        const int fb = p2g_terms(sc, mp, a.s.mass[p], a.s.mu0[p], a.s.lam0[p], Jp, F, C, v, fx, tm);
-        if (fb) atomicAdd(&wg_fb, 1u);
-        const int a0 = base[0] - O0, a1 = base[1] - O1, a2 = base[2] - O2;
-        if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5)
-          scatter_lds(acc_s, (a0 * 8 + a1) * 8 + a2, w, tm);
-        else
-          scatter_global(a, sc, base, w, tm);
+        for (int d = 0; d < 3; ++d) bspline(fx[r][d], w[d]);
+        if (p2g_terms(sc, mp, a.s.mass[p], a.s.mu0[p], a.s.lam0[p], Jp, F, C, v, fx[r], tm[r]))
+          atomicAdd(&wg_fb, 1u);
+        const int a0 = base[r][0] - O0, a1 = base[r][1] - O1, a2 = base[r][2] - O2;
+        if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5) {
+          act[r] = true;
+          float b[4];
+          contribution_bound(w, tm[r], b);
+          bool finite = true;
+#pragma unroll
+          for (int d = 0; d < 4; ++d) finite = finite && tfinite(b[d]);
+          if (finite) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) bsum[d] += b[d];
+          } else {  // NaN/Inf terms keep their value through the float path
+            act[r] = false;
+            scatter_global(a, sc, base[r], w, tm[r]);
+          }
+        } else {
+          scatter_global(a, sc, base[r], w, tm[r]);
+        }
       }
     }
     if constexpr (MODE != 2) {
+      // chunk-wide bound per channel -> fixed-point scales
+#pragma unroll
+      for (int d = 0; d < 4; ++d) {
+        const float m = warp_sum(bsum[d]);
+        if (lane == 0) red_s[wv][d] = m;
+      }
       __syncthreads();
+      if (t < 4) {
+        float m = 0.f;
+#pragma unroll
+        for (int w2 = 0; w2 < ENG_THREADS / 64; ++w2) m += red_s[w2][t];
+        m *= 1.0001f;  // round-up margin for the FP32 sums above
+        const float s = m > 0.f ? 1073741824.f / m : 1.f;  // 2^30 / bound
+        cs.s[t] = s;
+        cs.inv[t] = 1.0 / (double)s;
+      }
+      __syncthreads();
+      const float scl[4] = {cs.s[0], cs.s[1], cs.s[2], cs.s[3]};
+#pragma unroll
+      for (int r = 0; r < P; ++r) {
+        if (!act[r]) continue;
+        float w[3][3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) bspline(fx[r][d], w[d]);
+        const int a0 = base[r][0] - O0, a1 = base[r][1] - O1, a2 = base[r][2] - O2;
+        scatter_fixed(acc_s, (a0 * 8 + a1) * 8 + a2, w, tm[r], scl);
+      }
+      __syncthreads();
+      const double inv[4] = {cs.inv[0], cs.inv[1], cs.inv[2], cs.inv[3]};
       for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
-        const float m0 = acc_s[0][e], m1 = acc_s[1][e], m2 = acc_s[2][e], m3 = acc_s[3][e];
-        if (m0 != 0.f || m1 != 0.f || m2 != 0.f || m3 != 0.f) {
+        const int q0 = (int)acc_s[0][e], q1 = (int)acc_s[1][e], q2 = (int)acc_s[2][e],
+                  q3 = (int)acc_s[3][e];
+        if (q0 != 0 || q1 != 0 || q2 != 0 || q3 != 0) {
           const int gi = O0 + (e >> 6), gj = O1 + ((e >> 3) & 7), gk = O2 + (e & 7);
           if (gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res) {
             float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, gi, gj, gk));
-            atomicAdd(dst + 0, m0);
-            atomicAdd(dst + 1, m1);
-            atomicAdd(dst + 2, m2);
-            atomicAdd(dst + 3, m3);
+            atomicAdd(dst + 0, (float)((double)q0 * inv[0]));
+            atomicAdd(dst + 1, (float)((double)q1 * inv[1]));
+            atomicAdd(dst + 2, (float)((double)q2 * inv[2]));
+            atomicAdd(dst + 3, (float)((double)q3 * inv[3]));
             const int si = (gi >> 2) - (bi - 1), sj = (gj >> 2) - (bj - 1), sk = (gk >> 2) - (bk - 1);
             sub_active[(si * 3 + sj) * 3 + sk] = 1u;
           }
@@ -744,6 +862,7 @@ EngineArgs Engine::args(double* sums, int max_steps) const {
   a.list = d_list;
   a.counters = d_counters;
   a.chunks = d_chunks;
+  a.keys = d_sorted_keys;
   a.nchunks = d_nchunks;
   a.block_scene = d_block_scene;
   a.oob = d_oob;
@@ -807,6 +926,7 @@ void Engine::enqueue_bin(hipStream_t s) {
   exclusive_scan(d_cflags, d_cidx, n, d_part, nullptr, s);
   hipLaunchKernelGGL(chunk_write_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_cflags, d_cidx,
                      d_bend, d_block_scene, chunk, d_chunks, d_nchunks);
+  d_sorted_keys = ks;
 }
 
 void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
@@ -817,9 +937,22 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
     e1 = tevent();
     hipEventRecord(e0, s);
   }
-  if (mode == 0) hipLaunchKernelGGL(transfer_kernel<0>, dim3(g), dim3(ENG_THREADS), 0, s, a);
-  else if (mode == 1) hipLaunchKernelGGL(transfer_kernel<1>, dim3(g), dim3(ENG_THREADS), 0, s, a);
-  else hipLaunchKernelGGL(transfer_kernel<2>, dim3(g), dim3(ENG_THREADS), 0, s, a);
+#define MPM_LAUNCH_T(M, PV) \
+  hipLaunchKernelGGL((transfer_kernel<M, PV>), dim3(g), dim3(ENG_THREADS), 0, s, a)
+  if (chunk == 256) {
+    if (mode == 0) MPM_LAUNCH_T(0, 1);
+    else if (mode == 1) MPM_LAUNCH_T(1, 1);
+    else MPM_LAUNCH_T(2, 1);
+  } else if (chunk == 512) {
+    if (mode == 0) MPM_LAUNCH_T(0, 2);
+    else if (mode == 1) MPM_LAUNCH_T(1, 2);
+    else MPM_LAUNCH_T(2, 2);
+  } else {
+    if (mode == 0) MPM_LAUNCH_T(0, 4);
+    else if (mode == 1) MPM_LAUNCH_T(1, 4);
+    else MPM_LAUNCH_T(2, 4);
+  }
+#undef MPM_LAUNCH_T
   if (timing) {
     hipEventRecord(e1, s);
     tpairs.push_back({mode, e0, e1});

@@ -49,7 +49,8 @@ struct EngineArgs {
   uint32_t* flags;
   uint32_t* list;
   uint32_t* counters;  // [0] active blocks, [1] finished workgroups, [2] step
-  const int4* chunks;  // start, count, global block, scene
+  const int4* chunks;  // start, count, global block, scene (one per occupied block)
+  const uint32_t* keys;  // binned (sorted) cell keys of the current order
   const uint32_t* nchunks;
   const int* block_scene;
   unsigned long long* oob;

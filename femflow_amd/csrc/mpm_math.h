@@ -17,7 +17,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "../../include/mpm355.h"
+#include "mpm355.h"
 
 #define MPM_DEV __device__ __forceinline__
 
@@ -38,7 +38,12 @@ template <> MPM_DEV double tcbrt<double>(double x) { return cbrt(x); }
 template <typename T> MPM_DEV T trcbrt(T x);
 template <> MPM_DEV float trcbrt<float>(float x) { return rcbrtf(x); }
 template <> MPM_DEV double trcbrt<double>(double x) { return rcbrt(x); }
-template <typename T> MPM_DEV bool tfinite(T x) { return x - x == T(0); }
+// Finite test on the exponent bits (independent of how the compiler treats
+// NaN in floating-point comparisons).
+MPM_DEV bool tfinite(float x) { return (__float_as_uint(x) & 0x7f800000u) != 0x7f800000u; }
+MPM_DEV bool tfinite(double x) {
+  return ((unsigned long long)__double_as_longlong(x) & 0x7ff0000000000000ull) != 0x7ff0000000000000ull;
+}
 
 // Quadratic B-spline weights in the reference's per-index polynomial form
 // (three_d/p2g.py:55).  They equal N(i - fx) for fx in [0.5, 1.5); for
@@ -222,21 +227,60 @@ MPM_DEV void polar_svd_f64(const double* F, double R[9]) {
       R[3 * i + j] = U[3 * i] * V[3 * j] + U[3 * i + 1] * V[3 * j + 1] + U[3 * i + 2] * V[3 * j + 2];
 }
 
-// Polar factor in precision T with FP64 fallback.  Returns 1 when the
-// fallback ran (counted by callers for the conditioning report).
-template <typename T> MPM_DEV int polar3(const T* F, T R[9]);
-template <> MPM_DEV int polar3<float>(const float* F, float R[9]) {
-  if (polar_newton<float>(F, R, 12, 2e-7f)) return 0;
+// 3x3 matrices passed by value to the out-of-line FP64 fallbacks, which keeps
+// the rarely taken FP64 code (and its registers) out of the hot kernels.
+struct Mat3f {
+  float a[9];
+};
+struct Svd3f {
+  float U[9], s[3], V[9];
+};
+
+__device__ __noinline__ Mat3f polar_fallback_f64(Mat3f F) {
   double Fd[9], Rd[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) Fd[i] = F[i];
+  for (int i = 0; i < 9; ++i) Fd[i] = F.a[i];
   polar_svd_f64(Fd, Rd);
+  Mat3f R;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) R[i] = (float)Rd[i];
+  for (int i = 0; i < 9; ++i) R.a[i] = (float)Rd[i];
+  return R;
+}
+
+__device__ __noinline__ Svd3f svd_fallback_f64(Mat3f F) {
+  double Fd[9], Ud[9], sd[3], Vd[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fd[i] = F.a[i];
+  svd3<double>(Fd, Ud, sd, Vd, 10);
+  Svd3f r;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    r.U[i] = (float)Ud[i];
+    r.V[i] = (float)Vd[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i) r.s[i] = (float)sd[i];
+  return r;
+}
+
+// Polar factor in precision T with FP64 fallback.  Returns 1 when the
+// fallback ran (counted by callers for the conditioning report).  Newton
+// converges quadratically: a step change of 1e-5 (FP32) / 1e-9 (FP64) leaves
+// an error near 1e-10 / 1e-18, below the rounding floor of each type, while a
+// tolerance at the rounding floor itself would never be met reliably.
+template <typename T> MPM_DEV int polar3(const T* F, T R[9]);
+template <> MPM_DEV int polar3<float>(const float* F, float R[9]) {
+  if (polar_newton<float>(F, R, 12, 1e-5f)) return 0;
+  Mat3f Fm;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) Fm.a[i] = F[i];
+  const Mat3f Rm = polar_fallback_f64(Fm);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R[i] = Rm.a[i];
   return 1;
 }
 template <> MPM_DEV int polar3<double>(const double* F, double R[9]) {
-  if (polar_newton<double>(F, R, 40, 1e-15)) return 0;
+  if (polar_newton<double>(F, R, 40, 1e-9)) return 0;
   polar_svd_f64(F, R);
   return 1;
 }
@@ -247,14 +291,17 @@ template <typename T> MPM_DEV int svd3_checked(const T* F, T U[9], T s[3], T V[9
 template <> MPM_DEV int svd3_checked<float>(const float* F, float U[9], float s[3], float V[9]) {
   svd3<float>(F, U, s, V, 5);
   if (s[2] >= 1e-2f * s[0] && tfinite(s[0] + s[1] + s[2])) return 0;
-  double Fd[9], Ud[9], sd[3], Vd[9];
+  Mat3f Fm;
 #pragma unroll
-  for (int i = 0; i < 9; ++i) Fd[i] = F[i];
-  svd3<double>(Fd, Ud, sd, Vd, 10);
+  for (int i = 0; i < 9; ++i) Fm.a[i] = F[i];
+  const Svd3f r = svd_fallback_f64(Fm);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) { U[i] = (float)Ud[i]; V[i] = (float)Vd[i]; }
+  for (int i = 0; i < 9; ++i) {
+    U[i] = r.U[i];
+    V[i] = r.V[i];
+  }
 #pragma unroll
-  for (int i = 0; i < 3; ++i) s[i] = (float)sd[i];
+  for (int i = 0; i < 3; ++i) s[i] = r.s[i];
   return 1;
 }
 template <> MPM_DEV int svd3_checked<double>(const double* F, double U[9], double s[3], double V[9]) {

@@ -3,7 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "../../include/mpm355.h"
+#include "mpm355.h"
 
 #define MPM_PHASE_MAX_MATERIALS 16
 

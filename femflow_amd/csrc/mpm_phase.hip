@@ -231,14 +231,19 @@ __global__ void phase_g2p_kernel(PhaseArgs<T> a) {
       T U[9], s[3], V[9];
       svd3_checked<T>(Fn, U, s, V);
       const T oldJ = det3(Fn);
+      const T lo = T(1) - tc, hi = T(1) + ts;
+      // rebuild only when a singular value is actually clamped (otherwise
+      // U diag(s) V^T is F itself)
+      if (s[0] < lo || s[0] > hi || s[1] < lo || s[1] > hi || s[2] < lo || s[2] > hi) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) s[c] = tmin(tmax(s[c], T(1) - tc), T(1) + ts);
+        for (int c = 0; c < 3; ++c) s[c] = tmin(tmax(s[c], lo), hi);
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-          Fn[3 * i + j] = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
-                          U[3 * i + 2] * s[2] * V[3 * j + 2];
+          for (int j = 0; j < 3; ++j)
+            Fn[3 * i + j] = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
+                            U[3 * i + 2] * s[2] * V[3 * j + 2];
+      }
       const T det = det3(Fn) + T(1e-10);
       a.Jp[p] = tmin(tmax(a.Jp[p] * oldJ / det, T(0.6)), T(20));
     } else if (kind == MPM_WATER) {
@@ -277,18 +282,24 @@ __global__ void phase_g2p_kernel(PhaseArgs<T> a) {
                M[2] * Fp[1] + M[3] * Fp[3]};
 #pragma unroll
     for (int i = 0; i < 4; ++i) a.C[4 * p + i] = C[i];
-    // two_d/g2p.py:321-330: SVD rebuild and Jp update for every particle
-    T U[4], s[2], V[4];
-    svd2<T>(Fn, U, s, V);
+    // two_d/g2p.py:321-330: SVD rebuild and Jp update for every particle.
+    // U diag(s) V^T equals F exactly unless a singular value is clamped, so
+    // the rebuild (whose rounding would seed spurious stress) only runs then.
     const T oldJ = Fn[0] * Fn[3] - Fn[1] * Fn[2];
     if (kind == MPM_SNOW) {
-      s[0] = tmin(tmax(s[0], T(1) - tc), T(1) + ts);
-      s[1] = tmin(tmax(s[1], T(1) - tc), T(1) + ts);
+      T U[4], s[2], V[4];
+      svd2<T>(Fn, U, s, V);
+      const T lo = T(1) - tc, hi = T(1) + ts;
+      if (s[0] < lo || s[0] > hi || s[1] < lo || s[1] > hi) {
+        s[0] = tmin(tmax(s[0], lo), hi);
+        s[1] = tmin(tmax(s[1], lo), hi);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j)
+            Fn[2 * i + j] = U[2 * i] * s[0] * V[2 * j] + U[2 * i + 1] * s[1] * V[2 * j + 1];
+      }
     }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j) Fn[2 * i + j] = U[2 * i] * s[0] * V[2 * j] + U[2 * i + 1] * s[1] * V[2 * j + 1];
     const T det = Fn[0] * Fn[3] - Fn[1] * Fn[2] + T(1e-10);
     a.Jp[p] = tmin(tmax(a.Jp[p] * oldJ / det, T(0.6)), T(20));
 #pragma unroll

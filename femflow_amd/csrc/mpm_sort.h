@@ -4,7 +4,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
-#include "../../include/mpm355.h"
+#include "mpm355.h"
 
 namespace mpm {
 

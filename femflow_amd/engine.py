@@ -10,6 +10,7 @@ replayed.
 from __future__ import annotations
 
 import ctypes as ct
+import sys
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
@@ -80,6 +81,9 @@ class Engine:
         _lib.check(self._L.mpm_ctx_set_stream(self._h, self._stream()), "set_stream")
 
     def tune(self, chunk: int = 512, sort_every: int = 8, grid_mult: int = 4, use_graph: bool = True):
+        """chunk: binned particles per work item (256, 512, 1024); sort_every:
+        substeps between rebinning; grid_mult: persistent workgroups per CU;
+        use_graph: replay the substep sequence as a captured hipGraph."""
         self.config = dict(chunk=chunk, sort_every=sort_every, grid_mult=grid_mult, use_graph=use_graph)
         _lib.check(self._L.mpm_ctx_tune(self._h, chunk, sort_every, grid_mult, 1 if use_graph else 0),
                    "tune")
@@ -170,8 +174,6 @@ class Engine:
             self._h = None
 
     def __del__(self):
-        import sys
-
         if sys.is_finalizing():
             return
         try:
