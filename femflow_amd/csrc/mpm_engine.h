@@ -59,6 +59,7 @@ struct Engine {
   uint32_t *d_nchunks = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_cflags = nullptr,
            *d_cidx = nullptr, *d_part = nullptr;
   uint32_t* d_sorted_keys = nullptr;
+  uint32_t* d_hard = nullptr;
   void* d_ws = nullptr;
   size_t ws_bytes = 0;
 
@@ -66,13 +67,13 @@ struct Engine {
   void release();
   int alloc(void** p, size_t bytes);
   int init(int64_t n_max, int nscenes, const mpm_scene* sc, int nmat, const mpm_material* mats);
-  EngineArgs args(double* sums, int max_steps) const;
+  EngineArgs args(double* sums, int max_steps, int sel) const;
   int upload(const mpm_particles* p, const int32_t* scene_ids);
   int download(mpm_particles* p);
   void enqueue_bin(hipStream_t s);
   void enqueue_segment(int len, hipStream_t s, double* sums, int max_steps);
   void launch_transfer(int mode, hipStream_t s, const EngineArgs& a);
-  void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums);
+  void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k);
   hipEvent_t tevent();
   int run(int nsteps, double* sums);
   int take_counter(int which, int64_t* out);

@@ -14,6 +14,13 @@
 namespace mpm {
 
 // ------------------------------------------------------------------ G2P ----
+// Weight of stencil index i (uniform loop counter) along one axis, the same
+// arithmetic as bspline(): 0.5 (fx-1.5)^2, 0.75 - (fx-1)^2, 0.5 (fx-0.5)^2.
+__device__ __forceinline__ float weight_at(float fx, int i) {
+  const float t = fx - (i == 0 ? 1.5f : (i == 1 ? 1.0f : 0.5f));
+  return i == 1 ? 0.75f - t * t : 0.5f * t * t;
+}
+
 // v = sum w g; C = 4 inv_dx sum w g (ijk - fx)^T  (three_d/g2p.py:29-46),
 // accumulated as C[:, a] = 4 inv_dx (sum w g ijk_a - fx_a v).
 template <bool FROM_LDS>
@@ -40,27 +47,35 @@ __device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int aba
     }
   }
   const size_t gbase = (size_t)sc.block_base * 64;
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
+  // the outer axis stays a rolled loop: 9 node loads in flight instead of 27
+  // keeps the register footprint (and so the occupancy) of the callers sane
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const float wi = weight_at(fx[0], i);
+    const float fi = (float)i;
+    size_t Xi = 0;
+    if (!FROM_LDS) Xi = i == 0 ? X[0] : (i == 1 ? X[1] : X[2]);
+    const int ai = abase + i * 64;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const float wij = w[0][i] * w[1][j];
+      const float wij = wi * w[1][j];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         float4 g;
-        if (FROM_LDS) g = vel_s[abase + i * 64 + j * 8 + k];
-        else g = vel[gbase + X[i] + Y[j] + Z[k]];
+        if (FROM_LDS) g = vel_s[ai + j * 8 + k];
+        else g = vel[gbase + Xi + Y[j] + Z[k]];
         const float wt = wij * w[2][k];
         const float wg[3] = {wt * g.x, wt * g.y, wt * g.z};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           v[d] += wg[d];
-          if (i) S[0][d] += (float)i * wg[d];
+          S[0][d] += fi * wg[d];
           if (j) S[1][d] += (float)j * wg[d];
           if (k) S[2][d] += (float)k * wg[d];
         }
       }
     }
+  }
   const float c4 = 4.f * sc.inv_dx;
 #pragma unroll
   for (int d = 0; d < 3; ++d)
@@ -68,34 +83,67 @@ __device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int aba
     for (int a = 0; a < 3; ++a) C[3 * d + a] = c4 * (S[a][d] - fx[a] * v[d]);
 }
 
+// True when every singular value of F lies strictly inside [lo, hi]: the
+// eigenvalues of A = F^T F are then inside [lo^2, hi^2], i.e. A - lo^2 I and
+// hi^2 I - A are positive definite (leading principal minors > 0).
+__device__ __forceinline__ bool singular_values_inside(const float F[9], float lo, float hi) {
+  float A[6];  // symmetric: a00 a01 a02 a11 a12 a22
+  A[0] = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+  A[1] = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+  A[2] = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+  A[3] = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+  A[4] = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+  A[5] = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+  bool ok = true;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const float sg = side == 0 ? 1.f : -1.f;
+    const float sh = side == 0 ? lo * lo : hi * hi;
+    const float b00 = sg * (A[0] - sh), b11 = sg * (A[3] - sh), b22 = sg * (A[5] - sh);
+    const float b01 = sg * A[1], b02 = sg * A[2], b12 = sg * A[4];
+    const float m2 = b00 * b11 - b01 * b01;
+    const float m3 = b00 * (b11 * b22 - b12 * b12) - b01 * (b01 * b22 - b12 * b02) +
+                     b02 * (b01 * b12 - b11 * b02);
+    ok = ok && b00 > 0.f && m2 > 0.f && m3 > 0.f;
+  }
+  return ok;
+}
+
 // x += dt v, F <- (I + dt C) F, then plasticity (three_d/g2p.py:44-58).
-__device__ __forceinline__ int advance(const SceneDev& sc, const MatParams& mp, float x[3],
-                                       const float v[3], const float C[9], float F[9], float& Jp) {
-  int fb = 0;
+// Snow runs the FP32 SVD only when the cheap test above says a singular value
+// must be clamped.  Returns false when the clamp needs the FP64 SVD (F is
+// then left at the trial value and the particle goes to the deferred list).
+__device__ __forceinline__ bool advance(const SceneDev& sc, const MatParams& mp, float x[3],
+                                        const float v[3], const float C[9], float F[9], float& Jp) {
 #pragma unroll
   for (int d = 0; d < 3; ++d) x[d] += sc.dt * v[d];
   float M[9], Fn[9];
 #pragma unroll
   for (int i = 0; i < 9; ++i) M[i] = sc.dt * C[i] + ((i % 4 == 0) ? 1.f : 0.f);
   matmul3(M, F, Fn);
+  bool ok = true;
   if (mp.kind == MPM_SNOW) {
-    float U[9], s[3], V[9];
-    fb = svd3_checked<float>(Fn, U, s, V);
-    const float oldJ = det3(Fn);
     const float lo = 1.f - mp.theta_c, hi = 1.f + mp.theta_s;
-    // rebuild only when a singular value is clamped (U diag(s) V^T is F otherwise)
-    if (s[0] < lo || s[0] > hi || s[1] < lo || s[1] > hi || s[2] < lo || s[2] > hi) {
+    const float oldJ = det3(Fn);
+    if (!singular_values_inside(Fn, lo, hi)) {
+      float U[9], s[3], V[9];
+      svd3<float>(Fn, U, s, V, 5);
+      if (s[2] >= 1e-2f * s[0] && tfinite(s[0] + s[1] + s[2])) {
+        if (s[0] < lo || s[0] > hi || s[1] < lo || s[1] > hi || s[2] < lo || s[2] > hi) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) s[c] = tmin(tmax(s[c], lo), hi);
+          for (int c = 0; c < 3; ++c) s[c] = tmin(tmax(s[c], lo), hi);
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+          for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j)
-          Fn[3 * i + j] = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
-                          U[3 * i + 2] * s[2] * V[3 * j + 2];
+            for (int j = 0; j < 3; ++j)
+              Fn[3 * i + j] = U[3 * i] * s[0] * V[3 * j] + U[3 * i + 1] * s[1] * V[3 * j + 1] +
+                              U[3 * i + 2] * s[2] * V[3 * j + 2];
+        }
+      } else {
+        ok = false;
+      }
     }
-    const float det = det3(Fn) + 1e-10f;
-    Jp = tmin(tmax(Jp * oldJ / det, 0.6f), 20.f);
+    if (ok) Jp = tmin(tmax(Jp * oldJ / (det3(Fn) + 1e-10f), 0.6f), 20.f);
   } else if (mp.kind == MPM_WATER) {
     const float c = tcbrt(det3(Fn));
 #pragma unroll
@@ -103,7 +151,14 @@ __device__ __forceinline__ int advance(const SceneDev& sc, const MatParams& mp, 
   }
 #pragma unroll
   for (int i = 0; i < 9; ++i) F[i] = Fn[i];
-  return fb;
+  return ok;
+}
+
+// Append particle p to the deferred (FP64) list; bit 0 = plasticity pending,
+// bit 1 = no P2G wanted (the G2P-only pass).
+__device__ __forceinline__ void defer(const EngineArgs& a, int64_t p, uint32_t flags) {
+  const uint32_t i = atomicAdd(a.hard_count, 1u);
+  a.hard_list[i] = ((uint32_t)p << 2) | flags;
 }
 
 // ------------------------------------------------------------------ P2G ----
@@ -115,14 +170,38 @@ struct P2GTerms {
   float m;
 };
 
-__device__ __forceinline__ int p2g_terms(const SceneDev& sc, const MatParams& mp, float mass,
-                                         float mu0, float lam0, float Jp, const float F[9],
-                                         const float C[9], const float v[3], const float fx[3],
-                                         P2GTerms& t) {
-  float mu, lam, A[9];
+// FP32 terms; false when the FP32 polar factor is not trustworthy (F near
+// singular or the iteration did not settle): the caller defers the particle.
+__device__ __forceinline__ bool p2g_terms(const SceneDev& sc, const MatParams& mp, float mass,
+                                          float mu0, float lam0, float Jp, const float F[9],
+                                          const float C[9], const float v[3], const float fx[3],
+                                          P2GTerms& t) {
+  float mu, lam, R[9];
   lame<float>(mp.kind, mp.hardening, mu0, lam0, Jp, mu, lam);
-  const int fb = affine3<float>(F, C, mass, mu, lam, mp.kind, sc.inv_dx, sc.dt, sc.vol,
-                                sc.quirks == MPM_QUIRKS_TEXTBOOK, A);
+  if (mu != 0.f) {
+    if (!polar_newton<float>(F, R, 12, 1e-5f)) return false;
+  }
+  const float J = det3(F);
+  const float scale = -(sc.dt * sc.vol) * (4.f * sc.inv_dx * sc.inv_dx);
+  const float vt = lam * (J - 1.f) * J;
+  const bool diag_only = sc.quirks == MPM_QUIRKS_TEXTBOOK || mp.kind == MPM_WATER;
+  const float m2 = 2.f * mu;
+  float A[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float d0 = 0.f, d1 = 0.f, d2 = 0.f;
+    if (mu != 0.f) {
+      d0 = m2 * (F[3 * i] - R[3 * i]);
+      d1 = m2 * (F[3 * i + 1] - R[3 * i + 1]);
+      d2 = m2 * (F[3 * i + 2] - R[3 * i + 2]);
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      float pf = d0 * F[3 * j] + d1 * F[3 * j + 1] + d2 * F[3 * j + 2];
+      pf += (diag_only && i != j) ? 0.f : vt;
+      A[3 * i + j] = scale * pf + mass * C[3 * i + j];
+    }
+  }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     t.q0[d] = mass * v[d] - sc.dx * (A[3 * d] * fx[0] + A[3 * d + 1] * fx[1] + A[3 * d + 2] * fx[2]);
@@ -130,28 +209,7 @@ __device__ __forceinline__ int p2g_terms(const SceneDev& sc, const MatParams& mp
     for (int e = 0; e < 3; ++e) t.Adx[3 * d + e] = A[3 * d + e] * sc.dx;
   }
   t.m = mass;
-  return fb;
-}
-
-__device__ __forceinline__ void scatter_lds(float (*acc)[ARENA_NODES], int abase, const float w[3][3],
-                                            const P2GTerms& t) {
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const float wij = w[0][i] * w[1][j];
-      float qij[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) qij[d] = t.q0[d] + t.Adx[3 * d] * (float)i + t.Adx[3 * d + 1] * (float)j;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float wt = wij * w[2][k];
-        const int off = abase + i * 64 + j * 8 + k;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) atomicAdd(&acc[d][off], wt * (qij[d] + t.Adx[3 * d + 2] * (float)k));
-        atomicAdd(&acc[3][off], wt * t.m);
-      }
-    }
+  return true;
 }
 
 __device__ __forceinline__ void scatter_global(const EngineArgs& a, const SceneDev& sc,
@@ -240,34 +298,55 @@ __device__ __forceinline__ void contribution_bound(const float w[3][3], const P2
   b[3] = wm * fabsf(t.m);
 }
 
-__device__ __forceinline__ void scatter_fixed(uint32_t (*acc)[ARENA_NODES], int abase,
+// Accumulation arena: node (a0, a1, a2) of the 8x8x8 arena lives at
+// a0*69 + a1*9 + a2, which spreads the 64 cells of a block over the 64 LDS
+// banks with at most two per bank for every stencil offset (a plain 64/8/1
+// layout puts them on 16 banks).
+constexpr int ACC_S0 = 69, ACC_S1 = 9, ACC_NODES = 7 * ACC_S0 + 7 * ACC_S1 + 7 + 1;
+
+__device__ __forceinline__ void scatter_fixed(uint32_t (*acc)[ACC_NODES], int abase, float fx0,
                                               const float w[3][3], const P2GTerms& t, const float sc[4]) {
+  // terms pre-scaled to fixed point once per particle
+  float q0s[3], A0s[3], A1s[3], A2s[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i)
+  for (int d = 0; d < 3; ++d) {
+    q0s[d] = t.q0[d] * sc[d];
+    A0s[d] = t.Adx[3 * d] * sc[d];
+    A1s[d] = t.Adx[3 * d + 1] * sc[d];
+    A2s[d] = t.Adx[3 * d + 2] * sc[d];
+  }
+  const float ms = t.m * sc[3];
+#pragma unroll 1
+  for (int i = 0; i < 3; ++i) {
+    const float wi = weight_at(fx0, i);
+    const float fi = (float)i;
+    const int ai = abase + i * ACC_S0;
+    float qi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) qi[d] = q0s[d] + A0s[d] * fi;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const float wij = w[0][i] * w[1][j];
+      const float wij = wi * w[1][j];
       float qij[3];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) qij[d] = t.q0[d] + t.Adx[3 * d] * (float)i + t.Adx[3 * d + 1] * (float)j;
+      for (int d = 0; d < 3; ++d) qij[d] = qi[d] + A1s[d] * (float)j;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const float wt = wij * w[2][k];
-        const int off = abase + i * 64 + j * 8 + k;
+        const int off = ai + j * ACC_S1 + k;
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
-          const float val = d < 3 ? wt * (qij[d] + t.Adx[3 * d + 2] * (float)k) : wt * t.m;
-          const int q = __float2int_rn(val * sc[d]);
-          atomicAdd(&acc[d][off], (uint32_t)q);
-        }
+        for (int d = 0; d < 3; ++d)
+          atomicAdd(&acc[d][off], (uint32_t)__float2int_rn(wt * (qij[d] + A2s[d] * (float)k)));
+        atomicAdd(&acc[3][off], (uint32_t)__float2int_rn(wt * ms));
       }
     }
+  }
 }
 
 template <int MODE, int P>
 __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   __shared__ float4 vel_s[MODE == 0 ? 1 : ARENA_NODES];
-  __shared__ uint32_t acc_s[4][MODE == 2 ? 1 : ARENA_NODES];
+  __shared__ uint32_t acc_s[4][MODE == 2 ? 1 : ACC_NODES];
   __shared__ float red_s[ENG_THREADS / 64][4];
   __shared__ ChunkScale cs;
   __shared__ uint32_t sub_active[27];
@@ -291,13 +370,15 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
     const int nb = sc.nb;
     const int bi = local / (nb * nb), bj = (local / nb) % nb, bk = local % nb;
     const int O0 = bi * 4 - 1, O1 = bj * 4 - 1, O2 = bk * 4 - 1;
-    for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
-      if constexpr (MODE != 0) {
+    if constexpr (MODE != 0) {
+      for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
         const int gi = O0 + (e >> 6), gj = O1 + ((e >> 3) & 7), gk = O2 + (e & 7);
         const bool in = gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res;
         vel_s[e] = in ? a.vel[node_addr(sc, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
-      if constexpr (MODE != 2) {
+    }
+    if constexpr (MODE != 2) {
+      for (int e = t; e < ACC_NODES; e += ENG_THREADS) {
         acc_s[0][e] = 0u;
         acc_s[1][e] = 0u;
         acc_s[2][e] = 0u;
@@ -351,12 +432,16 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
           gather<true>(vel_s, (a0 * 8 + a1) * 8 + a2, a.vel, sc, base[r], w, fx[r], v, C);
         else
           gather<false>(vel_s, 0, a.vel, sc, base[r], w, fx[r], v, C);
-        if (advance(sc, mp, x, v, C, F, Jp)) atomicAdd(&wg_fb, 1u);
+        const bool ok = advance(sc, mp, x, v, C, F, Jp);
         store3(a.s.x, p, x);
         store3(a.s.v, p, v);
         store9(a.s.C, p, C);
         store9(a.s.F, p, F);
         a.s.Jp[p] = Jp;
+        if (!ok) {
+          defer(a, p, MODE == 2 ? 3u : 1u);
+          continue;
+        }
       }
       if constexpr (MODE != 2) {
         if (!locate3(x, sc, base[r], fx[r])) {
@@ -365,8 +450,10 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) bspline(fx[r][d], w[d]);
-        if (p2g_terms(sc, mp, a.s.mass[p], a.s.mu0[p], a.s.lam0[p], Jp, F, C, v, fx[r], tm[r]))
-          atomicAdd(&wg_fb, 1u);
+        if (!p2g_terms(sc, mp, a.s.mass[p], a.s.mu0[p], a.s.lam0[p], Jp, F, C, v, fx[r], tm[r])) {
+          defer(a, p, 0u);
+          continue;
+        }
         const int a0 = base[r][0] - O0, a1 = base[r][1] - O1, a2 = base[r][2] - O2;
         if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5) {
           act[r] = true;
@@ -413,13 +500,14 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) bspline(fx[r][d], w[d]);
         const int a0 = base[r][0] - O0, a1 = base[r][1] - O1, a2 = base[r][2] - O2;
-        scatter_fixed(acc_s, (a0 * 8 + a1) * 8 + a2, w, tm[r], scl);
+        scatter_fixed(acc_s, a0 * ACC_S0 + a1 * ACC_S1 + a2, fx[r][0], w, tm[r], scl);
       }
       __syncthreads();
       const double inv[4] = {cs.inv[0], cs.inv[1], cs.inv[2], cs.inv[3]};
       for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
-        const int q0 = (int)acc_s[0][e], q1 = (int)acc_s[1][e], q2 = (int)acc_s[2][e],
-                  q3 = (int)acc_s[3][e];
+        const int ph = (e >> 6) * ACC_S0 + ((e >> 3) & 7) * ACC_S1 + (e & 7);
+        const int q0 = (int)acc_s[0][ph], q1 = (int)acc_s[1][ph], q2 = (int)acc_s[2][ph],
+                  q3 = (int)acc_s[3][ph];
         if (q0 != 0 || q1 != 0 || q2 != 0 || q3 != 0) {
           const int gi = O0 + (e >> 6), gj = O1 + ((e >> 3) & 7), gk = O2 + (e & 7);
           if (gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res) {
@@ -449,6 +537,104 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   }
 }
 
+// ------------------------------------------------------- deferred (FP64) ----
+// Particles the FP32 transfer declined: snow plasticity whose SVD is too
+// ill-conditioned for FP32 (s_min < 1e-2 s_max), and stresses whose FP32
+// polar iteration did not settle (F near singular).  They are finished here
+// in FP64 (Jacobi SVD, Newton polar with SVD fallback) and scattered with
+// global atomics.  The last workgroup to finish clears the list.
+__global__ __launch_bounds__(ENG_THREADS) void deferred_kernel(EngineArgs a) {
+  const uint32_t count = a.counters[3];
+  uint32_t done_here = 0;
+  for (uint32_t i = blockIdx.x * ENG_THREADS + threadIdx.x; i < count; i += gridDim.x * ENG_THREADS) {
+    const uint32_t e = a.hard_list[i];
+    const int64_t p = e >> 2;
+    ++done_here;
+    const uint32_t attr = a.s.attr[p];
+    int mid = (int)(attr & 255u);
+    mid = mid < a.nmat ? mid : a.nmat - 1;
+    const MatParams mp = a.mats[mid];
+    const SceneDev sc = a.scenes[attr >> 8];
+    double F[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = a.s.F[9 * p + k];
+    double Jp = a.s.Jp[p];
+    if ((e & 1u) && mp.kind == MPM_SNOW) {
+      double U[9], sv[3], V[9];
+      svd3<double>(F, U, sv, V, 10);
+      const double oldJ = det3(F);
+      const double lo = 1.0 - (double)mp.theta_c, hi = 1.0 + (double)mp.theta_s;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sv[c] = tmin(tmax(sv[c], lo), hi);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          F[3 * r + c] = U[3 * r] * sv[0] * V[3 * c] + U[3 * r + 1] * sv[1] * V[3 * c + 1] +
+                         U[3 * r + 2] * sv[2] * V[3 * c + 2];
+      Jp = tmin(tmax(Jp * oldJ / (det3(F) + 1e-10), 0.6), 20.0);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) a.s.F[9 * p + k] = (float)F[k];
+      a.s.Jp[p] = (float)Jp;
+    }
+    if (e & 2u) continue;
+    float xf[3];
+    load3(a.s.x, p, xf);
+    int base[3];
+    float fxf[3];
+    if (!locate3(xf, sc, base, fxf)) {
+      atomicAdd(a.oob, 1ull);
+      continue;
+    }
+    double C[9], v[3], fx[3], w[3][3];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) C[k] = a.s.C[9 * p + k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      v[d] = a.s.v[3 * p + d];
+      fx[d] = fxf[d];
+      bspline(fx[d], w[d]);
+    }
+    const double m = a.s.mass[p];
+    double mu, lam, A[9];
+    lame<double>(mp.kind, (double)mp.hardening, (double)a.s.mu0[p], (double)a.s.lam0[p], Jp, mu, lam);
+    affine3<double>(F, C, m, mu, lam, mp.kind, (double)sc.inv_dx, (double)sc.dt, (double)sc.vol,
+                    sc.quirks == MPM_QUIRKS_TEXTBOOK, A);
+    const double dx = sc.dx;
+#pragma unroll
+    for (int i0 = 0; i0 < 3; ++i0)
+#pragma unroll
+      for (int j0 = 0; j0 < 3; ++j0)
+#pragma unroll
+        for (int k0 = 0; k0 < 3; ++k0) {
+          const double wt = w[0][i0] * w[1][j0] * w[2][k0];
+          const double dp[3] = {(i0 - fx[0]) * dx, (j0 - fx[1]) * dx, (k0 - fx[2]) * dx};
+          float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, base[0] + i0, base[1] + j0, base[2] + k0));
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            atomicAdd(dst + d, (float)(wt * (m * v[d] + A[3 * d] * dp[0] + A[3 * d + 1] * dp[1] + A[3 * d + 2] * dp[2])));
+          atomicAdd(dst + 3, (float)(wt * m));
+        }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int i = base[0] + ((c >> 2) & 1) * 2, j = base[1] + ((c >> 1) & 1) * 2, k = base[2] + (c & 1) * 2;
+      mark_active(a, (uint32_t)(sc.block_base + ((i >> 2) * sc.nb + (j >> 2)) * sc.nb + (k >> 2)));
+    }
+  }
+  if (done_here) atomicAdd(a.fallback, (unsigned long long)done_here);
+}
+
+// Start of a segment: clear both active-block list counters and the deferred
+// list, and advance the per-substep sums index by the segment length.
+__global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    counters[0] = 0u;
+    counters[1] = 0u;
+    counters[3] = 0u;
+    if (with_sums) counters[2] += (uint32_t)len;
+  }
+}
+
 // ------------------------------------------------------------ grid update ----
 // three_d/grid_op.py:25-47 on the active blocks only; also zeroes the
 // accumulation grid and the block flags for the next P2G, and (optionally)
@@ -459,9 +645,14 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-__global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, int with_sums) {
+__global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, int with_sums, int len,
+                                                                  int k) {
   __shared__ double red[ENG_THREADS / 64][8];
-  const uint32_t count = a.counters[0];
+  const uint32_t count = *a.list_count;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *a.other_count = 0u;  // the list the next substep appends to
+    a.counters[3] = 0u;   // deferred list (its pass has run)
+  }
   const int t = threadIdx.x;
   const int sub = t >> 6, inner = t & 63;
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -520,24 +711,14 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
     }
   }
   __syncthreads();
-  if (t == 0) {
-    if (with_sums) {
-      const uint32_t step = a.counters[2];
-      if ((int)step < a.max_steps) {
-        for (int c = 0; c < 7; ++c) {
-          double r = 0;
-          for (int w = 0; w < ENG_THREADS / 64; ++w) r += red[w][c];
-          atomicAdd(a.sums + (size_t)step * 8 + c, r);
-        }
+  if (t == 0 && with_sums) {
+    const int step = (int)a.counters[2] - len + k;
+    if (step >= 0 && step < a.max_steps) {
+      for (int c = 0; c < 7; ++c) {
+        double r = 0;
+        for (int w = 0; w < ENG_THREADS / 64; ++w) r += red[w][c];
+        atomicAdd(a.sums + (size_t)step * 8 + c, r);
       }
-    }
-    __threadfence();
-    const uint32_t prev = atomicAdd(a.counters + 1, 1u);
-    if (prev == gridDim.x - 1) {
-      a.counters[0] = 0u;
-      a.counters[1] = 0u;
-      if (with_sums) a.counters[2] += 1u;
-      __threadfence();
     }
   }
 }
@@ -823,7 +1004,7 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   if ((rc = alloc((void**)&d_acc, sizeof(float4) * 64 * blocks))) return rc;
   if ((rc = alloc((void**)&d_vel, sizeof(float4) * 64 * blocks))) return rc;
   if ((rc = alloc((void**)&d_flags, sizeof(uint32_t) * blocks))) return rc;
-  if ((rc = alloc((void**)&d_list, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_list, sizeof(uint32_t) * 2 * blocks))) return rc;
   if ((rc = alloc((void**)&d_bstart, sizeof(uint32_t) * blocks))) return rc;
   if ((rc = alloc((void**)&d_bend, sizeof(uint32_t) * blocks))) return rc;
   if ((rc = alloc((void**)&d_counters, sizeof(uint32_t) * 16))) return rc;
@@ -838,6 +1019,7 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   if ((rc = alloc((void**)&d_cflags, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_cidx, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_part, sizeof(uint32_t) * (n / 4096 + 4)))) return rc;
+  if ((rc = alloc((void**)&d_hard, sizeof(uint32_t) * n))) return rc;
   ws_bytes = radix_workspace_bytes(n);
   if ((rc = alloc(&d_ws, ws_bytes))) return rc;
   HIPCHK(hipMemset(d_acc, 0, sizeof(float4) * 64 * blocks));
@@ -850,7 +1032,7 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   return MPM_OK;
 }
 
-EngineArgs Engine::args(double* sums, int max_steps) const {
+EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   EngineArgs a{};
   a.s = st[cur];
   a.scenes = d_scenes;
@@ -859,13 +1041,17 @@ EngineArgs Engine::args(double* sums, int max_steps) const {
   a.acc = d_acc;
   a.vel = d_vel;
   a.flags = d_flags;
-  a.list = d_list;
+  a.list = d_list + (size_t)sel * (size_t)total_blocks;
+  a.list_count = d_counters + sel;
+  a.other_count = d_counters + (sel ^ 1);
   a.counters = d_counters;
   a.chunks = d_chunks;
   a.keys = d_sorted_keys;
   a.nchunks = d_nchunks;
   a.block_scene = d_block_scene;
   a.oob = d_oob;
+  a.hard_list = d_hard;
+  a.hard_count = d_counters + 3;
   a.fallback = d_fallback;
   a.sums = sums;
   a.max_steps = max_steps;
@@ -912,7 +1098,7 @@ void Engine::enqueue_bin(hipStream_t s) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_nchunks, (int64_t)1, 0u);
     return;
   }
-  EngineArgs a = args(nullptr, 0);
+  EngineArgs a = args(nullptr, 0, 0);
   hipLaunchKernelGGL(engine_keys_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, a, n, d_keys);
   uint32_t *ks, *vs;
   const uint32_t max_key = (uint32_t)(total_blocks * 64 - 1);
@@ -957,9 +1143,10 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
     hipEventRecord(e1, s);
     tpairs.push_back({mode, e0, e1});
   }
+  hipLaunchKernelGGL(deferred_kernel, dim3(num_cus), dim3(ENG_THREADS), 0, s, a);
 }
 
-void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums) {
+void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k) {
   const int g = num_cus * 2;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (timing) {
@@ -967,7 +1154,7 @@ void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums) {
     e1 = tevent();
     hipEventRecord(e0, s);
   }
-  hipLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, a, with_sums);
+  hipLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, a, with_sums, len, k);
   if (timing) {
     hipEventRecord(e1, s);
     tpairs.push_back({3, e0, e1});
@@ -985,16 +1172,17 @@ hipEvent_t Engine::tevent() {
 
 // One segment of `len` substeps: bin, P2G, grid, (G2P2G, grid) x (len-1), G2P.
 void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps) {
-  enqueue_bin(s);
-  EngineArgs a = args(sums, max_steps);
   const int ws = sums ? 1 : 0;
-  launch_transfer(0, s, a);
-  launch_grid(s, a, ws);
+  hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws);
+  enqueue_bin(s);
+  // substep k appends to active list k % 2; its grid update clears the other
+  launch_transfer(0, s, args(sums, max_steps, 0));
+  launch_grid(s, args(sums, max_steps, 0), ws, len, 0);
   for (int k = 1; k < len; ++k) {
-    launch_transfer(1, s, a);
-    launch_grid(s, a, ws);
+    launch_transfer(1, s, args(sums, max_steps, k & 1));
+    launch_grid(s, args(sums, max_steps, k & 1), ws, len, k);
   }
-  launch_transfer(2, s, a);
+  launch_transfer(2, s, args(sums, max_steps, len & 1));
 }
 
 int Engine::run(int nsteps, double* sums) {

@@ -47,8 +47,13 @@ struct EngineArgs {
   float4* acc;
   float4* vel;
   uint32_t* flags;
-  uint32_t* list;
-  uint32_t* counters;  // [0] active blocks, [1] finished workgroups, [2] step
+  uint32_t* list;        // active blocks appended by this substep's P2G
+  uint32_t* list_count;  // their number
+  uint32_t* other_count; // the other list's count (cleared by the grid update)
+  uint32_t* counters;  // [0], [1] active-block list counts, [2] sums step base,
+                       // [3] deferred particles
+  uint32_t* hard_list;   // deferred particles: index << 2 | plasticity pending | no P2G
+  uint32_t* hard_count;  // == counters + 3
   const int4* chunks;  // start, count, global block, scene (one per occupied block)
   const uint32_t* keys;  // binned (sorted) cell keys of the current order
   const uint32_t* nchunks;
@@ -83,7 +88,7 @@ __device__ __forceinline__ bool locate3(const float* xp, const SceneDev& sc, int
 __device__ __forceinline__ void mark_active(const EngineArgs& a, uint32_t g) {
   if (__hip_atomic_load(a.flags + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
     if (atomicOr(a.flags + g, 1u) == 0u) {
-      const uint32_t idx = atomicAdd(a.counters, 1u);
+      const uint32_t idx = atomicAdd(a.list_count, 1u);
       a.list[idx] = g;
     }
   }
