@@ -224,6 +224,30 @@ int32_t mpm_run(mpm_ctx* ctx, int32_t nsteps, double* sums) {
   return ctx->eng.run(nsteps, sums);
 }
 
+int32_t mpm_run_frames(mpm_ctx* ctx, int32_t nsteps, float* frames) {
+  if (!ctx || !frames) {
+    set_error("mpm_run_frames: null argument");
+    return MPM_ERR_ARG;
+  }
+  mpm::Engine& e = ctx->eng;
+  return e.run(nsteps, nullptr, frames);
+}
+
+int32_t mpm_state_positions(mpm_ctx* ctx, float* x) {
+  if (!ctx) {
+    set_error("mpm_state_positions: null context");
+    return MPM_ERR_ARG;
+  }
+  mpm::Engine& e = ctx->eng;
+  CHECK_HIP(hipEventRecord(e.ev_in, e.user_stream));
+  CHECK_HIP(hipStreamWaitEvent(e.own_stream, e.ev_in, 0));
+  const int rc = e.positions(x);
+  if (rc) return rc;
+  CHECK_HIP(hipEventRecord(e.ev_out, e.own_stream));
+  CHECK_HIP(hipStreamWaitEvent(e.user_stream, e.ev_out, 0));
+  return MPM_OK;
+}
+
 int32_t mpm_substep(mpm_ctx* ctx, mpm_particles* p, const int32_t* scene_ids, int32_t nsteps) {
   int rc;
   if ((rc = mpm_state_upload(ctx, p, scene_ids))) return rc;

@@ -71,11 +71,14 @@ struct Engine {
   int upload(const mpm_particles* p, const int32_t* scene_ids);
   int download(mpm_particles* p);
   void enqueue_bin(hipStream_t s);
-  void enqueue_segment(int len, hipStream_t s, double* sums, int max_steps);
+  void enqueue_segment(int len, hipStream_t s, double* sums, int max_steps, float* frames = nullptr,
+                       int64_t step0 = 0);
+  void enqueue_frame(hipStream_t s, float* frames, int64_t step);
+  int positions(float* x);
   void launch_transfer(int mode, hipStream_t s, const EngineArgs& a);
   void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k);
   hipEvent_t tevent();
-  int run(int nsteps, double* sums);
+  int run(int nsteps, double* sums, float* frames = nullptr);
   int take_counter(int which, int64_t* out);
   int particle_sums(double* out);
   int kernel_times(double* ms, int64_t* counts);

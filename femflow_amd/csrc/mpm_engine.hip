@@ -860,6 +860,15 @@ __global__ void download_kernel(EngineState s, int64_t n, float* __restrict__ x,
   Jp[o] = s.Jp[i];
 }
 
+// Positions in the caller's order (one frame of the rollout output).
+__global__ void positions_kernel(EngineState s, int64_t n, float* __restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t o = s.order[i];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) x[3 * o + d] = s.x[3 * i + d];
+}
+
 __global__ void particle_sums_kernel(EngineState s, int64_t n, double* out) {
   __shared__ double red[ENG_THREADS / 64][8];
   double acc[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -1171,7 +1180,14 @@ hipEvent_t Engine::tevent() {
 }
 
 // One segment of `len` substeps: bin, P2G, grid, (G2P2G, grid) x (len-1), G2P.
-void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps) {
+void Engine::enqueue_frame(hipStream_t s, float* frames, int64_t step) {
+  if (!frames || n <= 0) return;
+  hipLaunchKernelGGL(positions_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, st[cur], n,
+                     frames + (size_t)step * 3 * (size_t)n);
+}
+
+void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps, float* frames,
+                             int64_t step0) {
   const int ws = sums ? 1 : 0;
   hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws);
   enqueue_bin(s);
@@ -1180,12 +1196,14 @@ void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps
   launch_grid(s, args(sums, max_steps, 0), ws, len, 0);
   for (int k = 1; k < len; ++k) {
     launch_transfer(1, s, args(sums, max_steps, k & 1));
+    enqueue_frame(s, frames, step0 + k - 1);  // substep k-1 finished its G2P
     launch_grid(s, args(sums, max_steps, k & 1), ws, len, k);
   }
   launch_transfer(2, s, args(sums, max_steps, len & 1));
+  enqueue_frame(s, frames, step0 + len - 1);
 }
 
-int Engine::run(int nsteps, double* sums) {
+int Engine::run(int nsteps, double* sums, float* frames) {
   if (nsteps < 0) {
     set_error("mpm_run: nsteps must be >= 0");
     return MPM_ERR_ARG;
@@ -1201,7 +1219,7 @@ int Engine::run(int nsteps, double* sums) {
   int done = 0;
   while (done < nsteps) {
     const int len = (nsteps - done) < sort_every ? (nsteps - done) : sort_every;
-    const bool graphed = use_graph && !timing && n > 0;
+    const bool graphed = use_graph && !timing && n > 0 && !frames;
     if (graphed) {
       const int parity = cur;
       const GraphKey key{len, parity, (uintptr_t)sums, sums ? nsteps : 0};
@@ -1222,13 +1240,23 @@ int Engine::run(int nsteps, double* sums) {
       HIPCHK(hipGraphLaunch(exec, own_stream));
       cur = parity ^ 1;
     } else {
-      enqueue_segment(len, own_stream, sums, nsteps);
+      enqueue_segment(len, own_stream, sums, nsteps, frames, done);
     }
     HIPCHK(hipGetLastError());
     done += len;
   }
   HIPCHK(hipEventRecord(ev_out, own_stream));
   HIPCHK(hipStreamWaitEvent(user_stream, ev_out, 0));
+  return MPM_OK;
+}
+
+int Engine::positions(float* x) {
+  if (!x) {
+    set_error("mpm_state_positions: null output");
+    return MPM_ERR_ARG;
+  }
+  enqueue_frame(own_stream, x, 0);
+  HIPCHK(hipGetLastError());
   return MPM_OK;
 }
 

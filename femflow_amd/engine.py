@@ -142,6 +142,28 @@ class Engine:
             if n:
                 raise RuntimeError(f"{n} particle stencil(s) left the grid")
 
+    def run_frames(self, nsteps: int, frames: torch.Tensor, check_oob: bool = True):
+        """Advance ``nsteps`` substeps, writing each substep's positions (caller
+        order) into ``frames``, a float32 CUDA tensor (nsteps, N, 3)."""
+        self._check_tensor(frames, "frames")
+        if frames.numel() < nsteps * self.n * 3:
+            raise ValueError("frames needs nsteps * N * 3 entries")
+        self._sync_stream()
+        _lib.check(self._L.mpm_run_frames(self._h, int(nsteps), frames.data_ptr()), "run_frames")
+        if check_oob:
+            n = self.oob_count()
+            if n:
+                raise RuntimeError(f"{n} particle stencil(s) left the grid")
+
+    def positions(self, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Current positions in the caller's order, (N, 3) float32 on the device."""
+        if out is None:
+            out = torch.empty((self.n, 3), dtype=torch.float32, device=self.device)
+        self._check_tensor(out, "out")
+        self._sync_stream()
+        _lib.check(self._L.mpm_state_positions(self._h, out.data_ptr()), "positions")
+        return out
+
     def oob_count(self) -> int:
         out = ct.c_int64()
         _lib.check(self._L.mpm_ctx_oob_count(self._h, ct.byref(out)), "oob_count")

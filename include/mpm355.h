@@ -160,6 +160,12 @@ int32_t mpm_state_download(mpm_ctx* ctx, mpm_particles* p);
  * nsteps * 8 entries) each substep writes FP64 [grid mass, grid momentum x3
  * after P2G, grid momentum x3 after the update, particle-count] (see docs). */
 int32_t mpm_run(mpm_ctx* ctx, int32_t nsteps, double* sums);
+/* Like mpm_run (no sums), also writing each substep's positions in the
+ * caller's order to frames (device float, nsteps * n * 3): the per-step frame
+ * record of simulation.py:140-142 (map_particles_to_pos, particle.py:30-33). */
+int32_t mpm_run_frames(mpm_ctx* ctx, int32_t nsteps, float* frames);
+/* Current positions in the caller's order (device float, n * 3). */
+int32_t mpm_state_positions(mpm_ctx* ctx, float* x);
 /* Convenience for the reference's per-call pattern: upload, run, download. */
 int32_t mpm_substep(mpm_ctx* ctx, mpm_particles* p, const int32_t* scene_ids, int32_t nsteps);
 /* Particles whose stencil left their grid since the context was created or
