@@ -1,0 +1,71 @@
+"""Batched scenes in one launch sequence, and the autotuner."""
+import json
+
+import numpy as np
+import pytest
+
+from femflow_amd.scenes import batched_scene_specs
+
+
+def test_batched_specs_ranges_and_determinism():
+    a = batched_scene_specs(n_scenes=64, seed=0)
+    b = batched_scene_specs(n_scenes=64, seed=0)
+    assert a == b
+    ns = np.array([s["n"] for s in a])
+    assert ns.min() >= 4096 and ns.max() <= 262144
+    assert all(64 <= s["res"] <= 128 for s in a)
+    assert all(0.1 <= s["side"] <= 0.3 for s in a)
+    assert {s["material"] for s in a} == {0, 1, 2}
+    # log-uniform: the median sits near the geometric mean of the range
+    assert 20000 < np.median(ns) < 60000
+
+
+@pytest.mark.gpu
+def test_batched_scenes_match_oracle_per_scene(gpu):
+    from femflow_amd.batched import BatchedRun, build_batched
+    from oracle import mpm_oracle as O
+    from tests._parity import field_errors
+
+    work = build_batched(n_scenes=12, seed=3, count_scale=0.03, keep_states=True)
+    run = BatchedRun(work, device=gpu)
+    run.engine.tune(chunk=512, sort_every=4)
+    run.run(6)
+    out = run.download()
+    off = 0
+    for i, (st, pr) in enumerate(zip(work["states"], work["params"])):
+        n = len(st["x"])
+        ref = O.copy_state(st)
+        for k in ("x", "v", "F", "C", "Jp", "mass", "mu0", "lam0"):
+            src = run.t[k][off:off + n].double().cpu().numpy()
+            ref[k] = src.reshape(st[k].shape)
+        for _ in range(6):
+            O.substep(ref, pr)
+        errs = field_errors({k: v[off:off + n] for k, v in out.items()}, ref, pr["inv_dx"])
+        print("scene", i, "res", pr["res"], "n", n, errs, flush=True)
+        for k in ("x", "v", "F"):
+            assert errs[k] < 1e-4, (i, errs)
+        off += n
+    run.close()
+
+
+@pytest.mark.gpu
+def test_autotune_restores_state_and_caches(gpu, tmp_path):
+    import torch
+
+    from femflow_amd.batched import BatchedRun, build_batched
+
+    work = build_batched(n_scenes=6, seed=5, count_scale=0.05)
+    run = BatchedRun(work, device=gpu)
+    before = run.download()
+    cache = tmp_path / "tune.json"
+    res = run.autotune(steps=8, reps=1, candidates=[(256, 4, 8), (512, 8, 8)], cache_path=str(cache))
+    after = run.download()
+    for k in before:
+        assert torch.equal(before[k], after[k]), k
+    assert res["config"] in ({"chunk": 256, "grid_mult": 4, "sort_every": 8},
+                             {"chunk": 512, "grid_mult": 8, "sort_every": 8})
+    data = json.loads(cache.read_text())
+    assert len(data) == 1
+    again = run.autotune(cache_path=str(cache))
+    assert again["cached"] and again["config"] == res["config"]
+    run.close()
