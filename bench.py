@@ -49,6 +49,8 @@ def parse():
                     help="substeps run with per-kernel HIP-event timing")
     ap.add_argument("--batched-steps", type=int, default=500,
                     help="substeps of the 64-scene batched workload (0 skips it)")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="first run 5 plain device copies of this many MiB (calibrates HBM byte counters)")
     return ap.parse_args()
 
 
@@ -116,6 +118,15 @@ def main():
     from femflow_amd.scenes import MATERIALS_C2, scene_3d_cubes
 
     dev = torch.device("cuda", 0)
+    if args.copy_mb > 0:
+        src = torch.ones(args.copy_mb << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copies: 5 x {args.copy_mb} MiB read + {args.copy_mb} MiB written",
+              file=sys.stderr, flush=True)
+        del src, dst
     st, pr = scene_3d_cubes(n=args.n, res=args.res)
     n = len(st["x"])
     t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float32, device=dev)  # noqa: E731

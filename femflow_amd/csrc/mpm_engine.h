@@ -60,6 +60,9 @@ struct Engine {
            *d_cidx = nullptr, *d_part = nullptr;
   uint32_t* d_sorted_keys = nullptr;
   uint32_t* d_hard = nullptr;
+  float4* d_stage = nullptr;
+  uint32_t stage_cap = 0;
+  uint32_t *d_bfirst = nullptr, *d_bcount = nullptr;
   void* d_ws = nullptr;
   size_t ws_bytes = 0;
 

@@ -240,21 +240,18 @@ __device__ __forceinline__ void scatter_global(const EngineArgs& a, const SceneD
 }
 
 // ----------------------------------------------------------- particle I/O ----
-__device__ __forceinline__ void load3(const float* p, int64_t i, float o[3]) {
+// One address per particle and field; components at immediate offsets.
+template <int K>
+__device__ __forceinline__ void loadk(const float* f, int64_t p, float o[K]) {
+  const float* b = f + tix(p, K, 0);
 #pragma unroll
-  for (int d = 0; d < 3; ++d) o[d] = p[3 * i + d];
+  for (int d = 0; d < K; ++d) o[d] = b[d * PTILE];
 }
-__device__ __forceinline__ void load9(const float* p, int64_t i, float o[9]) {
+template <int K>
+__device__ __forceinline__ void storek(float* f, int64_t p, const float o[K]) {
+  float* b = f + tix(p, K, 0);
 #pragma unroll
-  for (int d = 0; d < 9; ++d) o[d] = p[9 * i + d];
-}
-__device__ __forceinline__ void store3(float* p, int64_t i, const float o[3]) {
-#pragma unroll
-  for (int d = 0; d < 3; ++d) p[3 * i + d] = o[d];
-}
-__device__ __forceinline__ void store9(float* p, int64_t i, const float o[9]) {
-#pragma unroll
-  for (int d = 0; d < 9; ++d) p[9 * i + d] = o[d];
+  for (int d = 0; d < K; ++d) b[d * PTILE] = o[d];
 }
 
 // --------------------------------------------------------- transfer kernel ----
@@ -304,53 +301,28 @@ __device__ __forceinline__ void contribution_bound(const float w[3][3], const P2
 // layout puts them on 16 banks).
 constexpr int ACC_S0 = 69, ACC_S1 = 9, ACC_NODES = 7 * ACC_S0 + 7 * ACC_S1 + 7 + 1;
 
-__device__ __forceinline__ void scatter_fixed(uint32_t (*acc)[ACC_NODES], int abase, float fx0,
-                                              const float w[3][3], const P2GTerms& t, const float sc[4]) {
-  // terms pre-scaled to fixed point once per particle
-  float q0s[3], A0s[3], A1s[3], A2s[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    q0s[d] = t.q0[d] * sc[d];
-    A0s[d] = t.Adx[3 * d] * sc[d];
-    A1s[d] = t.Adx[3 * d + 1] * sc[d];
-    A2s[d] = t.Adx[3 * d + 2] * sc[d];
-  }
-  const float ms = t.m * sc[3];
-#pragma unroll 1
-  for (int i = 0; i < 3; ++i) {
-    const float wi = weight_at(fx0, i);
-    const float fi = (float)i;
-    const int ai = abase + i * ACC_S0;
-    float qi[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) qi[d] = q0s[d] + A0s[d] * fi;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const float wij = wi * w[1][j];
-      float qij[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) qij[d] = qi[d] + A1s[d] * (float)j;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const float wt = wij * w[2][k];
-        const int off = ai + j * ACC_S1 + k;
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          atomicAdd(&acc[d][off], (uint32_t)__float2int_rn(wt * (qij[d] + A2s[d] * (float)k)));
-        atomicAdd(&acc[3][off], (uint32_t)__float2int_rn(wt * ms));
-      }
-    }
-  }
-}
-
 template <int MODE, int P>
 __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
-  __shared__ float4 vel_s[MODE == 0 ? 1 : ARENA_NODES];
-  __shared__ uint32_t acc_s[4][MODE == 2 ? 1 : ACC_NODES];
+  // The velocity arena (read by the G2P half) and the accumulation arena
+  // (written by the P2G half after every gather of the chunk is done) share
+  // one LDS region.
+  constexpr int VEL_WORDS = MODE == 0 ? 4 : 4 * ARENA_NODES;
+  constexpr int ACC_WORDS = MODE == 2 ? 4 : 4 * ACC_NODES;
+  __shared__ __align__(16) uint32_t arena_s[VEL_WORDS > ACC_WORDS ? VEL_WORDS : ACC_WORDS];
+  float4* const vel_s = reinterpret_cast<float4*>(arena_s);
+  uint32_t(*const acc_s)[MODE == 2 ? 1 : ACC_NODES] =
+      reinterpret_cast<uint32_t(*)[MODE == 2 ? 1 : ACC_NODES]>(arena_s);
   __shared__ float red_s[ENG_THREADS / 64][4];
   __shared__ ChunkScale cs;
   __shared__ uint32_t sub_active[27];
   __shared__ uint32_t wg_oob, wg_fb;
+  // P2G term table (fx[3], q0[3], Adx[9], m per chunk entry) and the arena
+  // cell of each entry (-1: not in the table).  Entry q sits at slot
+  // q + q/16, which spreads the scatter phase's strided reads over the banks.
+  constexpr int TAB = MODE == 2 ? 1 : P * ENG_THREADS;
+  constexpr int TABS = TAB + TAB / 16;
+  __shared__ float pt_s[16][TABS];
+  __shared__ int pb_s[TABS];
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
   if (t == 0) {
@@ -377,66 +349,48 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         vel_s[e] = in ? a.vel[node_addr(sc, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
-    if constexpr (MODE != 2) {
-      for (int e = t; e < ACC_NODES; e += ENG_THREADS) {
-        acc_s[0][e] = 0u;
-        acc_s[1][e] = 0u;
-        acc_s[2][e] = 0u;
-        acc_s[3][e] = 0u;
-      }
-    }
     if (t < 27) sub_active[t] = 0;
     __syncthreads();
-    // Particle q of the chunk for (thread, r).  P2G modes space the lanes of
-    // a wave S = ceil(count/64) particles apart so that lanes rarely share a
-    // cell (the same LDS addresses) in one atomic; G2P-only is coalesced.
-    const int S = (ch.y + 63) >> 6;
-    const int PP = (S + 3) >> 2;  // <= P
-    P2GTerms tm[P];
-    int base[P][3];
-    float fx[P][3];
-    bool act[P];
+    // Phase 1: particle q = r * ENG_THREADS + t of the chunk (coalesced).
+    // G2P modes gather, advance and store the particle; P2G modes then
+    // compute its terms into the LDS term table (or send it down the global
+    // path when it left the arena or its terms are not finite).
     float bsum[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
+#pragma unroll 1
     for (int r = 0; r < P; ++r) {
-      act[r] = false;
-      int q;
-      if constexpr (MODE == 2) {
-        q = r * ENG_THREADS + t;
-        if (q >= ch.y) continue;
-      } else {
-        const int jj = wv * PP + r;
-        q = lane * S + jj;
-        if (r >= PP || jj >= S || q >= ch.y) continue;
-      }
+      const int q = r * ENG_THREADS + t;
+      if (q >= ch.y) continue;
+      const int qs = q + (q >> 4);
+      if constexpr (MODE != 2) pb_s[qs] = -1;
       const int64_t p = (int64_t)ch.x + q;
-      float x[3], F[9], v[3], C[9], Jp, w[3][3];
-      load3(a.s.x, p, x);
-      load9(a.s.F, p, F);
+      float x[3], F[9], v[3], C[9], Jp, w[3][3], fx[3];
+      int base[3];
+      loadk<3>(a.s.x, p, x);
+      loadk<9>(a.s.F, p, F);
       Jp = a.s.Jp[p];
       int mid = (int)(a.s.attr[p] & 255u);
       mid = mid < a.nmat ? mid : a.nmat - 1;
       const MatParams mp = a.mats[mid];
       if constexpr (MODE == 0) {
-        load3(a.s.v, p, v);
-        load9(a.s.C, p, C);
+        loadk<3>(a.s.v, p, v);
+        loadk<9>(a.s.C, p, C);
       } else {
-        if (!locate3(x, sc, base[r], fx[r])) {
+        if (!locate3(x, sc, base, fx)) {
           atomicAdd(&wg_oob, 1u);
           continue;
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) bspline(fx[r][d], w[d]);
-        const int a0 = base[r][0] - O0, a1 = base[r][1] - O1, a2 = base[r][2] - O2;
+        for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
+        const int a0 = base[0] - O0, a1 = base[1] - O1, a2 = base[2] - O2;
         if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5)
-          gather<true>(vel_s, (a0 * 8 + a1) * 8 + a2, a.vel, sc, base[r], w, fx[r], v, C);
+          gather<true>(vel_s, (a0 * 8 + a1) * 8 + a2, a.vel, sc, base, w, fx, v, C);
         else
-          gather<false>(vel_s, 0, a.vel, sc, base[r], w, fx[r], v, C);
+          gather<false>(vel_s, 0, a.vel, sc, base, w, fx, v, C);
         const bool ok = advance(sc, mp, x, v, C, F, Jp);
-        store3(a.s.x, p, x);
-        store3(a.s.v, p, v);
-        store9(a.s.C, p, C);
-        store9(a.s.F, p, F);
+        storek<3>(a.s.x, p, x);
+        storek<3>(a.s.v, p, v);
+        storek<9>(a.s.C, p, C);
+        storek<9>(a.s.F, p, F);
         a.s.Jp[p] = Jp;
         if (!ok) {
           defer(a, p, MODE == 2 ? 3u : 1u);
@@ -444,34 +398,42 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         }
       }
       if constexpr (MODE != 2) {
-        if (!locate3(x, sc, base[r], fx[r])) {
+        if (!locate3(x, sc, base, fx)) {
           atomicAdd(&wg_oob, 1u);
           continue;
         }
 #pragma unroll
-        for (int d = 0; d < 3; ++d) bspline(fx[r][d], w[d]);
-        if (!p2g_terms(sc, mp, a.s.mass[p], a.s.mu0[p], a.s.lam0[p], Jp, F, C, v, fx[r], tm[r])) {
+        for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
+        P2GTerms tm;
+        if (!p2g_terms(sc, mp, a.s.mass[p], a.s.mu0[p], a.s.lam0[p], Jp, F, C, v, fx, tm)) {
           defer(a, p, 0u);
           continue;
         }
-        const int a0 = base[r][0] - O0, a1 = base[r][1] - O1, a2 = base[r][2] - O2;
-        if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5) {
-          act[r] = true;
+        const int a0 = base[0] - O0, a1 = base[1] - O1, a2 = base[2] - O2;
+        bool inside = a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5;
+        if (inside) {
           float b[4];
-          contribution_bound(w, tm[r], b);
+          contribution_bound(w, tm, b);
           bool finite = true;
 #pragma unroll
           for (int d = 0; d < 4; ++d) finite = finite && tfinite(b[d]);
           if (finite) {
 #pragma unroll
             for (int d = 0; d < 4; ++d) bsum[d] += b[d];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+              pt_s[d][qs] = fx[d];
+              pt_s[3 + d][qs] = tm.q0[d];
+            }
+#pragma unroll
+            for (int e = 0; e < 9; ++e) pt_s[6 + e][qs] = tm.Adx[e];
+            pt_s[15][qs] = tm.m;
+            pb_s[qs] = a0 * ACC_S0 + a1 * ACC_S1 + a2;
           } else {  // NaN/Inf terms keep their value through the float path
-            act[r] = false;
-            scatter_global(a, sc, base[r], w, tm[r]);
+            inside = false;
           }
-        } else {
-          scatter_global(a, sc, base[r], w, tm[r]);
         }
+        if (!inside) scatter_global(a, sc, base, w, tm);
       }
     }
     if constexpr (MODE != 2) {
@@ -482,6 +444,12 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         if (lane == 0) red_s[wv][d] = m;
       }
       __syncthreads();
+      for (int e = t; e < ACC_NODES; e += ENG_THREADS) {
+        acc_s[0][e] = 0u;
+        acc_s[1][e] = 0u;
+        acc_s[2][e] = 0u;
+        acc_s[3][e] = 0u;
+      }
       if (t < 4) {
         float m = 0.f;
 #pragma unroll
@@ -492,30 +460,84 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         cs.inv[t] = 1.0 / (double)s;
       }
       __syncthreads();
+      // Phase 2: scatter.  Particle q of the table for (lane, r), the lanes of
+      // a wave S = ceil(count/64) entries apart, so that they rarely share a
+      // cell; lanes that do share one start the 3x9-node stencil at different
+      // node slabs (q mod 3), so their atomics hit different arena words.
       const float scl[4] = {cs.s[0], cs.s[1], cs.s[2], cs.s[3]};
+      const int S = (ch.y + 63) >> 6;
+      const int PP = (S + 3) >> 2;  // <= P
+#pragma unroll 1
+      for (int r = 0; r < PP; ++r) {
+        const int jj = wv * PP + r;
+        const int q = lane * S + jj;
+        if (jj >= S || q >= ch.y) continue;
+        const int qs = q + (q >> 4);
+        const int pb = pb_s[qs];
+        if (pb < 0) continue;
+        const float fx0 = pt_s[0][qs];
+        float w1[3], w2[3];
+        bspline(pt_s[1][qs], w1);
+        bspline(pt_s[2][qs], w2);
+        float q0s[3], A0s[3], A1s[3], A2s[3];
 #pragma unroll
-      for (int r = 0; r < P; ++r) {
-        if (!act[r]) continue;
-        float w[3][3];
+        for (int d = 0; d < 3; ++d) {
+          q0s[d] = pt_s[3 + d][qs] * scl[d];
+          A0s[d] = pt_s[6 + 3 * d][qs] * scl[d];
+          A1s[d] = pt_s[7 + 3 * d][qs] * scl[d];
+          A2s[d] = pt_s[8 + 3 * d][qs] * scl[d];
+        }
+        const float ms = pt_s[15][qs] * scl[3];
+        const int i0 = q % 3;
+#pragma unroll 1
+        for (int sl = 0; sl < 3; ++sl) {
+          const int i = i0 + sl >= 3 ? i0 + sl - 3 : i0 + sl;
+          const float wi = weight_at(fx0, i);
+          const float fi = (float)i;
+          const int ai = pb + i * ACC_S0;
+          float qi[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) bspline(fx[r][d], w[d]);
-        const int a0 = base[r][0] - O0, a1 = base[r][1] - O1, a2 = base[r][2] - O2;
-        scatter_fixed(acc_s, a0 * ACC_S0 + a1 * ACC_S1 + a2, fx[r][0], w, tm[r], scl);
+          for (int d = 0; d < 3; ++d) qi[d] = q0s[d] + A0s[d] * fi;
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const float wij = wi * w1[j];
+            float qij[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) qij[d] = qi[d] + A1s[d] * (float)j;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+              const float wt = wij * w2[k];
+              const int off = ai + j * ACC_S1 + k;
+#pragma unroll
+              for (int d = 0; d < 3; ++d)
+                atomicAdd(&acc_s[d][off], (uint32_t)__float2int_rn(wt * (qij[d] + A2s[d] * (float)k)));
+              atomicAdd(&acc_s[3][off], (uint32_t)__float2int_rn(wt * ms));
+            }
+          }
+        }
       }
       __syncthreads();
       const double inv[4] = {cs.inv[0], cs.inv[1], cs.inv[2], cs.inv[3]};
+      const bool staged = c < a.stage_cap;
+      float4* stage = a.stage + (size_t)c * ARENA_NODES;
+#pragma unroll 1
       for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
         const int ph = (e >> 6) * ACC_S0 + ((e >> 3) & 7) * ACC_S1 + (e & 7);
         const int q0 = (int)acc_s[0][ph], q1 = (int)acc_s[1][ph], q2 = (int)acc_s[2][ph],
                   q3 = (int)acc_s[3][ph];
+        const float4 val = make_float4((float)((double)q0 * inv[0]), (float)((double)q1 * inv[1]),
+                                       (float)((double)q2 * inv[2]), (float)((double)q3 * inv[3]));
+        if (staged) stage[e] = val;
         if (q0 != 0 || q1 != 0 || q2 != 0 || q3 != 0) {
           const int gi = O0 + (e >> 6), gj = O1 + ((e >> 3) & 7), gk = O2 + (e & 7);
           if (gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res) {
-            float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, gi, gj, gk));
-            atomicAdd(dst + 0, (float)((double)q0 * inv[0]));
-            atomicAdd(dst + 1, (float)((double)q1 * inv[1]));
-            atomicAdd(dst + 2, (float)((double)q2 * inv[2]));
-            atomicAdd(dst + 3, (float)((double)q3 * inv[3]));
+            if (!staged) {
+              float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, gi, gj, gk));
+              atomicAdd(dst + 0, val.x);
+              atomicAdd(dst + 1, val.y);
+              atomicAdd(dst + 2, val.z);
+              atomicAdd(dst + 3, val.w);
+            }
             const int si = (gi >> 2) - (bi - 1), sj = (gj >> 2) - (bj - 1), sk = (gk >> 2) - (bk - 1);
             sub_active[(si * 3 + sj) * 3 + sk] = 1u;
           }
@@ -557,7 +579,7 @@ __global__ __launch_bounds__(ENG_THREADS) void deferred_kernel(EngineArgs a) {
     const SceneDev sc = a.scenes[attr >> 8];
     double F[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) F[k] = a.s.F[9 * p + k];
+    for (int k = 0; k < 9; ++k) F[k] = a.s.F[tix(p, 9, k)];
     double Jp = a.s.Jp[p];
     if ((e & 1u) && mp.kind == MPM_SNOW) {
       double U[9], sv[3], V[9];
@@ -574,12 +596,12 @@ __global__ __launch_bounds__(ENG_THREADS) void deferred_kernel(EngineArgs a) {
                          U[3 * r + 2] * sv[2] * V[3 * c + 2];
       Jp = tmin(tmax(Jp * oldJ / (det3(F) + 1e-10), 0.6), 20.0);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) a.s.F[9 * p + k] = (float)F[k];
+      for (int k = 0; k < 9; ++k) a.s.F[tix(p, 9, k)] = (float)F[k];
       a.s.Jp[p] = (float)Jp;
     }
     if (e & 2u) continue;
     float xf[3];
-    load3(a.s.x, p, xf);
+    loadk<3>(a.s.x, p, xf);
     int base[3];
     float fxf[3];
     if (!locate3(xf, sc, base, fxf)) {
@@ -588,10 +610,10 @@ __global__ __launch_bounds__(ENG_THREADS) void deferred_kernel(EngineArgs a) {
     }
     double C[9], v[3], fx[3], w[3][3];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) C[k] = a.s.C[9 * p + k];
+    for (int k = 0; k < 9; ++k) C[k] = a.s.C[tix(p, 9, k)];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      v[d] = a.s.v[3 * p + d];
+      v[d] = a.s.v[tix(p, 3, d)];
       fx[d] = fxf[d];
       bspline(fx[d], w[d]);
     }
@@ -648,66 +670,108 @@ __device__ __forceinline__ double wave_sum(double v) {
 __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, int with_sums, int len,
                                                                   int k) {
   __shared__ double red[ENG_THREADS / 64][8];
+  __shared__ float4 part[ENG_THREADS / 64 - 1][64];
   const uint32_t count = *a.list_count;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *a.other_count = 0u;  // the list the next substep appends to
     a.counters[3] = 0u;   // deferred list (its pass has run)
   }
   const int t = threadIdx.x;
-  const int sub = t >> 6, inner = t & 63;
+  const int grp = t >> 6, inner = t & 63;
+  constexpr int NG = ENG_THREADS / 64;
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t b = blockIdx.x * 4u + (uint32_t)sub; b < count; b += gridDim.x * 4u) {
+  // One active block per workgroup iteration: node `inner`, and the four
+  // waves split the staged chunk arenas (chunk index mod 4) before an LDS sum.
+  for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
     const uint32_t g = a.list[b];
     const SceneDev sc = a.scenes[a.block_scene[g]];
     const int local = (int)g - sc.block_base;
     const int nb = sc.nb;
-    const int i = (local / (nb * nb)) * 4 + (inner >> 4);
-    const int j = ((local / nb) % nb) * 4 + ((inner >> 2) & 3);
-    const int k = (local % nb) * 4 + (inner & 3);
+    const int bi = local / (nb * nb), bj = (local / nb) % nb, bk = local % nb;
+    const int o0 = inner >> 4, o1 = (inner >> 2) & 3, o2 = inner & 3;
+    const int i = bi * 4 + o0, j = bj * 4 + o1, k = bk * 4 + o2;
     const size_t addr = (size_t)g * 64 + inner;
-    const float4 q = a.acc[addr];
-    const float m = q.w;
-    float v[3] = {q.x, q.y, q.z};
-    if (with_sums) {
-      s[0] += m;
-      s[1] += q.x;
-      s[2] += q.y;
-      s[3] += q.z;
+    float4 q = grp == 0 ? a.acc[addr] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // Staged chunk arenas that cover this node: per axis, node offset o < 3
+    // lies in the arenas of blocks b-1 and b, o = 3 in those of b and b+1
+    // (an arena spans nodes 4b-1 .. 4b+6).
+#pragma unroll 1
+    for (int sn = 0; sn < 8; ++sn) {
+      const int s0 = (sn >> 2) - (o0 == 3 ? 0 : 1);
+      const int s1 = ((sn >> 1) & 1) - (o1 == 3 ? 0 : 1);
+      const int s2 = (sn & 1) - (o2 == 3 ? 0 : 1);
+      const int S0 = bi + s0, S1 = bj + s1, S2 = bk + s2;
+      if (S0 < 0 || S0 >= nb || S1 < 0 || S1 >= nb || S2 < 0 || S2 >= nb) continue;
+      const uint32_t S = (uint32_t)(sc.block_base + (S0 * nb + S1) * nb + S2);
+      const uint32_t cn = a.bcount[S];
+      if (cn == 0u) continue;
+      const uint32_t c0 = a.bfirst[S];
+      const uint32_t c1 = c0 + cn < a.stage_cap ? c0 + cn : a.stage_cap;
+      const float4* src = a.stage + ((o0 + 1 - 4 * s0) * 8 + (o1 + 1 - 4 * s1)) * 8 + (o2 + 1 - 4 * s2);
+#pragma unroll 4
+      for (uint32_t c = c0 + (uint32_t)grp; c < c1; c += NG) {
+        const float4 sv = src[(size_t)c * ARENA_NODES];
+        q.x += sv.x;
+        q.y += sv.y;
+        q.z += sv.z;
+        q.w += sv.w;
+      }
     }
-    if (m > 0.f) {
-      const float im = 1.f / m;
-      v[0] *= im;
-      v[1] *= im;
-      v[2] *= im;
-      v[1] += sc.dt * sc.gravity;
+    if (grp > 0) part[grp - 1][inner] = q;
+    __syncthreads();
+    if (grp == 0) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d) v[d] = tmin(tmax(v[d], -sc.v_allowed), sc.v_allowed);
-    }
-    const int I[3] = {i, j, k};
-    if (sc.boundary == MPM_BC_REFERENCE) {
+      for (int gg = 0; gg < NG - 1; ++gg) {
+        const float4 pv = part[gg][inner];
+        q.x += pv.x;
+        q.y += pv.y;
+        q.z += pv.z;
+        q.w += pv.w;
+      }
+      const float m = q.w;
+      float v[3] = {q.x, q.y, q.z};
+      if (with_sums) {
+        s[0] += m;
+        s[1] += q.x;
+        s[2] += q.y;
+        s[3] += q.z;
+      }
+      if (m > 0.f) {
+        const float im = 1.f / m;
+        v[0] *= im;
+        v[1] *= im;
+        v[2] *= im;
+        v[1] += sc.dt * sc.gravity;
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        if (I[d] < 1 || I[d] >= sc.res - 1) v[d] = 0.f;
-    } else {
+        for (int d = 0; d < 3; ++d) v[d] = tmin(tmax(v[d], -sc.v_allowed), sc.v_allowed);
+      }
+      const int I[3] = {i, j, k};
+      if (sc.boundary == MPM_BC_REFERENCE) {
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-        if (I[d] < sc.bound || I[d] > sc.res - sc.bound) v[0] = v[1] = v[2] = 0.f;
+        for (int d = 0; d < 3; ++d)
+          if (I[d] < 1 || I[d] >= sc.res - 1) v[d] = 0.f;
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          if (I[d] < sc.bound || I[d] > sc.res - sc.bound) v[0] = v[1] = v[2] = 0.f;
+      }
+      if (i > sc.res || j > sc.res || k > sc.res) v[0] = v[1] = v[2] = 0.f;
+      a.vel[addr] = make_float4(v[0], v[1], v[2], m);
+      a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (with_sums) {
+        s[4] += (double)m * v[0];
+        s[5] += (double)m * v[1];
+        s[6] += (double)m * v[2];
+      }
+      if (inner == 0) a.flags[g] = 0u;
     }
-    if (i > sc.res || j > sc.res || k > sc.res) v[0] = v[1] = v[2] = 0.f;
-    a.vel[addr] = make_float4(v[0], v[1], v[2], m);
-    a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (with_sums) {
-      s[4] += (double)m * v[0];
-      s[5] += (double)m * v[1];
-      s[6] += (double)m * v[2];
-    }
-    if (inner == 0) a.flags[g] = 0u;
+    __syncthreads();
   }
   if (with_sums) {
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
       const double r = wave_sum(s[c]);
-      if ((t & 63) == 0) red[sub][c] = r;
+      if ((t & 63) == 0) red[grp][c] = r;
     }
   }
   __syncthreads();
@@ -737,9 +801,9 @@ __global__ void engine_keys_kernel(EngineArgs a, int64_t n, uint32_t* __restrict
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const SceneDev sc = a.scenes[a.s.attr[i] >> 8];
-  const int b0 = cell_of(a.s.x[3 * i], sc.inv_dx, sc.res);
-  const int b1 = cell_of(a.s.x[3 * i + 1], sc.inv_dx, sc.res);
-  const int b2 = cell_of(a.s.x[3 * i + 2], sc.inv_dx, sc.res);
+  const int b0 = cell_of(a.s.x[tix(i, 3, 0)], sc.inv_dx, sc.res);
+  const int b1 = cell_of(a.s.x[tix(i, 3, 1)], sc.inv_dx, sc.res);
+  const int b2 = cell_of(a.s.x[tix(i, 3, 2)], sc.inv_dx, sc.res);
   const uint32_t blk = (uint32_t)(sc.block_base + ((b0 >> 2) * sc.nb + (b1 >> 2)) * sc.nb + (b2 >> 2));
   keys[i] = blk * 64u + (uint32_t)(((b0 & 3) << 4) | ((b1 & 3) << 2) | (b2 & 3));
 }
@@ -751,13 +815,13 @@ __global__ void reorder_kernel(EngineState src, EngineState dst, const uint32_t*
   const int64_t s = perm[i];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    dst.x[3 * i + d] = src.x[3 * s + d];
-    dst.v[3 * i + d] = src.v[3 * s + d];
+    dst.x[tix(i, 3, d)] = src.x[tix(s, 3, d)];
+    dst.v[tix(i, 3, d)] = src.v[tix(s, 3, d)];
   }
 #pragma unroll
   for (int d = 0; d < 9; ++d) {
-    dst.F[9 * i + d] = src.F[9 * s + d];
-    dst.C[9 * i + d] = src.C[9 * s + d];
+    dst.F[tix(i, 9, d)] = src.F[tix(s, 9, d)];
+    dst.C[tix(i, 9, d)] = src.C[tix(s, 9, d)];
   }
   dst.Jp[i] = src.Jp[s];
   dst.mass[i] = src.mass[s];
@@ -787,7 +851,8 @@ __global__ void chunk_flags_kernel(const uint32_t* __restrict__ keys, int64_t n,
 __global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
                                    const uint32_t* __restrict__ flags, const uint32_t* __restrict__ idx,
                                    const uint32_t* __restrict__ bend, const int* __restrict__ block_scene,
-                                   int chunk, int4* __restrict__ chunks, uint32_t* __restrict__ nchunks) {
+                                   int chunk, int4* __restrict__ chunks, uint32_t* __restrict__ nchunks,
+                                   uint32_t* __restrict__ bfirst, uint32_t* __restrict__ bcount) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   if (flags[i]) {
@@ -795,6 +860,10 @@ __global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
     const uint32_t cnt = bend[blk] - (uint32_t)i;
     chunks[idx[i]] = make_int4((int)i, (int)(cnt < (uint32_t)chunk ? cnt : (uint32_t)chunk), (int)blk,
                                block_scene[blk]);
+    if (i == 0 || (keys[i - 1] >> 6) != blk) {  // first chunk of the block
+      bfirst[blk] = idx[i];
+      bcount[blk] = (cnt + (uint32_t)chunk - 1u) / (uint32_t)chunk;
+    }
   }
   if (i == n - 1) *nchunks = idx[i] + flags[i];
 }
@@ -821,13 +890,13 @@ __global__ void upload_kernel(EngineState s, int64_t n, const float* __restrict_
   if (i >= n) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    s.x[3 * i + d] = x[3 * i + d];
-    s.v[3 * i + d] = v[3 * i + d];
+    s.x[tix(i, 3, d)] = x[3 * i + d];
+    s.v[tix(i, 3, d)] = v[3 * i + d];
   }
 #pragma unroll
   for (int d = 0; d < 9; ++d) {
-    s.F[9 * i + d] = F[9 * i + d];
-    s.C[9 * i + d] = C[9 * i + d];
+    s.F[tix(i, 9, d)] = F[9 * i + d];
+    s.C[tix(i, 9, d)] = C[9 * i + d];
   }
   s.Jp[i] = Jp[i];
   const int64_t mi = mat_scalar ? 0 : i;
@@ -849,13 +918,13 @@ __global__ void download_kernel(EngineState s, int64_t n, float* __restrict__ x,
   const int64_t o = s.order[i];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    x[3 * o + d] = s.x[3 * i + d];
-    v[3 * o + d] = s.v[3 * i + d];
+    x[3 * o + d] = s.x[tix(i, 3, d)];
+    v[3 * o + d] = s.v[tix(i, 3, d)];
   }
 #pragma unroll
   for (int d = 0; d < 9; ++d) {
-    F[9 * o + d] = s.F[9 * i + d];
-    C[9 * o + d] = s.C[9 * i + d];
+    F[9 * o + d] = s.F[tix(i, 9, d)];
+    C[9 * o + d] = s.C[tix(i, 9, d)];
   }
   Jp[o] = s.Jp[i];
 }
@@ -866,7 +935,7 @@ __global__ void positions_kernel(EngineState s, int64_t n, float* __restrict__ x
   if (i >= n) return;
   const int64_t o = s.order[i];
 #pragma unroll
-  for (int d = 0; d < 3; ++d) x[3 * o + d] = s.x[3 * i + d];
+  for (int d = 0; d < 3; ++d) x[3 * o + d] = s.x[tix(i, 3, d)];
 }
 
 __global__ void particle_sums_kernel(EngineState s, int64_t n, double* out) {
@@ -877,8 +946,8 @@ __global__ void particle_sums_kernel(EngineState s, int64_t n, double* out) {
     const double m = s.mass[i];
     acc[0] += m;
     for (int d = 0; d < 3; ++d) {
-      acc[1 + d] += m * (double)s.v[3 * i + d];
-      acc[4 + d] += m * (double)s.x[3 * i + d];
+      acc[1 + d] += m * (double)s.v[tix(i, 3, d)];
+      acc[4 + d] += m * (double)s.x[tix(i, 3, d)];
     }
   }
   const int t = threadIdx.x;
@@ -937,8 +1006,9 @@ int Engine::alloc(void** p, size_t bytes) {
 }
 
 int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, const mpm_material* mats) {
-  if (n_max_ < 0 || nscenes <= 0 || nscenes > MAX_SCENES || !sc || nmat_ <= 0 || nmat_ > 256 || !mats) {
-    set_error("mpm_ctx_create: bad argument (n_max >= 0, 1 <= nscenes <= 4096, 1 <= nmat <= 256)");
+  if (n_max_ < 0 || n_max_ >= (int64_t(1) << 30) || nscenes <= 0 || nscenes > MAX_SCENES || !sc || nmat_ <= 0 ||
+      nmat_ > 256 || !mats) {
+    set_error("mpm_ctx_create: bad argument (0 <= n_max < 2^30, 1 <= nscenes <= 4096, 1 <= nmat <= 256)");
     return MPM_ERR_ARG;
   }
   n_max = n_max_;
@@ -985,7 +1055,7 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   total_blocks = blocks;
   scenes_host = hs;
   int rc;
-  const int64_t n = n_max > 0 ? n_max : 1;
+  const int64_t n = ((n_max > 0 ? n_max : 1) + PTILE - 1) / PTILE * PTILE;
   if ((rc = alloc((void**)&d_scenes, sizeof(SceneDev) * nscenes))) return rc;
   if ((rc = alloc((void**)&d_mats, sizeof(MatParams) * nmat))) return rc;
   if ((rc = alloc((void**)&d_block_scene, sizeof(int) * blocks))) return rc;
@@ -1023,6 +1093,15 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   const int64_t max_chunks = n / 256 + blocks + 1;
   if ((rc = alloc((void**)&d_chunks, sizeof(int4) * (max_chunks < n + 1 ? max_chunks : n + 1)))) return rc;
   if ((rc = alloc((void**)&d_nchunks, sizeof(uint32_t) * 4))) return rc;
+  // staged P2G arenas: room for every chunk at the smallest chunk size unless
+  // nearly every particle sits in its own block
+  const int64_t cap_chunks = (max_chunks < n + 1 ? max_chunks : n + 1);
+  const int64_t want = 2 * ((n + 255) / 256) + 8192;
+  stage_cap = (uint32_t)(want < cap_chunks ? want : cap_chunks);
+  if ((rc = alloc((void**)&d_stage, sizeof(float4) * ARENA_NODES * (size_t)stage_cap))) return rc;
+  if ((rc = alloc((void**)&d_bfirst, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_bcount, sizeof(uint32_t) * blocks))) return rc;
+  HIPCHK(hipMemset(d_bcount, 0, sizeof(uint32_t) * blocks));
   if ((rc = alloc((void**)&d_keys, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_vals, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_cflags, sizeof(uint32_t) * n))) return rc;
@@ -1064,6 +1143,10 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.fallback = d_fallback;
   a.sums = sums;
   a.max_steps = max_steps;
+  a.stage = d_stage;
+  a.stage_cap = stage_cap;
+  a.bfirst = d_bfirst;
+  a.bcount = d_bcount;
   return a;
 }
 
@@ -1119,8 +1202,10 @@ void Engine::enqueue_bin(hipStream_t s) {
   hipLaunchKernelGGL(chunk_flags_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, chunk,
                      d_cflags);
   exclusive_scan(d_cflags, d_cidx, n, d_part, nullptr, s);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(total_blocks, 256)), dim3(256), 0, s, d_bcount,
+                     (int64_t)total_blocks, 0u);
   hipLaunchKernelGGL(chunk_write_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_cflags, d_cidx,
-                     d_bend, d_block_scene, chunk, d_chunks, d_nchunks);
+                     d_bend, d_block_scene, chunk, d_chunks, d_nchunks, d_bfirst, d_bcount);
   d_sorted_keys = ks;
 }
 
@@ -1156,7 +1241,7 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
 }
 
 void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k) {
-  const int g = num_cus * 2;
+  const int g = num_cus * 8;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (timing) {
     e0 = tevent();

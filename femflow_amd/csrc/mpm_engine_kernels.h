@@ -26,6 +26,17 @@ struct SceneDev {
   float inv_dx, dx, dt, gravity, vol, v_allowed;
 };
 
+// Particle state in tiles of 64 slots: inside a tile, component d of a
+// K-component field (x, v: K = 3; F, C: K = 9 row-major) occupies 64
+// consecutive floats, so one wave's access to one component of 64 slots is
+// contiguous, while one particle's components sit at a fixed 256-byte stride
+// from a single address.  Scalar fields are plain arrays.  Allocations are
+// padded to a whole number of tiles.
+constexpr int PTILE = 64;
+__host__ __device__ __forceinline__ int64_t tix(int64_t i, int K, int d) {
+  return ((i >> 6) * K + d) * PTILE + (i & 63);
+}
+
 struct EngineState {
   float* x;
   float* v;
@@ -62,6 +73,15 @@ struct EngineArgs {
   unsigned long long* fallback;
   double* sums;
   int max_steps;
+  // Per-chunk P2G results: chunk c < stage_cap writes its whole 8x8x8 arena
+  // (FP32 momentum + mass, arena order) to stage[c * 512 ...] with plain
+  // stores; the grid update sums, for each node, the arenas of the chunks of
+  // the blocks around it (first chunk and chunk count per block).  Chunks at
+  // or beyond stage_cap add into `acc` with global atomics instead.
+  float4* stage;
+  uint32_t stage_cap;
+  const uint32_t* bfirst;
+  const uint32_t* bcount;
 };
 
 __device__ __forceinline__ size_t node_addr(const SceneDev& sc, int i, int j, int k) {
