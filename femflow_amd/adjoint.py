@@ -1,0 +1,168 @@
+"""Differentiable rollouts of the explicit substep (hand-written adjoint).
+
+``rollout(x0, v0, mu0, lam0, mass, cfg)`` runs ``cfg.nsteps`` elastic
+(fixed-corotated) 3D substeps of mls_mpm.py:40-79 on a dense (res+1)^3 grid
+in FP32 on the GPU and returns the final positions and velocities.  It is a
+``torch.autograd.Function``: the backward pass is the library's adjoint of
+one substep (``mpm_adjoint_step``), applied in reverse order.  The forward
+keeps a checkpoint of the particle state every ``cfg.checkpoint_every``
+substeps; the backward replays one checkpoint interval at a time, keeping
+that interval's per-substep states (24 floats per particle per substep), and
+reverses it.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
+or scalar); mass is a constant.
+
+The reference has no gradients; the forward it differentiates is its step
+(three_d/p2g.py:14-80, three_d/grid_op.py:5-47, three_d/g2p.py:9-59 with the
+stress of utils.py:95-135 and constant hardening, utils.py:7-24).
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass
+from typing import Optional, Tuple
+
+import torch
+
+from . import _lib
+from ._arrays import stream_ptr
+
+
+@dataclass
+class RolloutConfig:
+    res: int
+    dt: float
+    volume: float
+    nsteps: int
+    gravity: float = -9.8
+    hardening: float = 1.0
+    checkpoint_every: int = 50
+    boundary: str = "reference"
+    bound: int = 3
+    quirks: str = "reference"
+
+    def params(self) -> _lib.Params:
+        inv_dx = float(self.res)
+        return _lib.Params(3, int(self.res), inv_dx, 1.0 / inv_dx, float(self.dt), float(self.volume),
+                           float(self.gravity),
+                           _lib.MPM_BC_REFERENCE if self.boundary == "reference" else _lib.MPM_BC_STICKY,
+                           int(self.bound),
+                           _lib.MPM_QUIRKS_REFERENCE if self.quirks == "reference" else _lib.MPM_QUIRKS_TEXTBOOK,
+                           1)
+
+
+def _f32(t: torch.Tensor, device) -> torch.Tensor:
+    return t.detach().to(device=device, dtype=torch.float32).contiguous()
+
+
+class DenseStepper:
+    """The dense-grid substep and its adjoint for one particle count."""
+
+    def __init__(self, cfg: RolloutConfig, n: int, device):
+        _lib.require_gpu()
+        self.L = _lib.load()
+        self.cfg = cfg
+        self.n = int(n)
+        self.device = torch.device(device)
+        self.prm = cfg.params()
+        nbytes = ct.c_int64()
+        _lib.check(self.L.mpm_dense_workspace_bytes(ct.byref(self.prm), ct.byref(nbytes)), "workspace")
+        self.work = torch.empty((nbytes.value + 3) // 4, dtype=torch.float32, device=self.device)
+        self.oob = torch.zeros(1, dtype=torch.int64, device=self.device)
+        self.Jp = torch.ones(max(self.n, 1), dtype=torch.float32, device=self.device)
+
+    def _particles(self, x, v, F, C, mass, mu0, lam0) -> _lib.Particles:
+        scalar = mass.numel() == 1 and mu0.numel() == 1 and lam0.numel() == 1
+        if not scalar and not (mass.numel() == mu0.numel() == lam0.numel() == self.n):
+            raise ValueError("mass, mu0 and lambda0 must all be scalars or all have one entry per particle")
+        return _lib.Particles(self.n, _lib.MPM_F32, 1 if scalar else 0, x.data_ptr(), v.data_ptr(),
+                              F.data_ptr(), C.data_ptr(), self.Jp.data_ptr(), mass.data_ptr(), mu0.data_ptr(),
+                              lam0.data_ptr(), None)
+
+    def step(self, x, v, F, C, mass, mu0, lam0):
+        p = self._particles(x, v, F, C, mass, mu0, lam0)
+        _lib.check(self.L.mpm_dense_substep(stream_ptr(self.device), ct.byref(self.prm), ct.byref(p),
+                                            float(self.cfg.hardening), self.work.data_ptr(),
+                                            self.oob.data_ptr()), "mpm_dense_substep")
+
+    def adjoint(self, x, v, F, C, mass, mu0, lam0, xb, vb, Fb, Cb, mub, lamb):
+        p = self._particles(x, v, F, C, mass, mu0, lam0)
+        _lib.check(self.L.mpm_adjoint_step(stream_ptr(self.device), ct.byref(self.prm), ct.byref(p),
+                                           float(self.cfg.hardening), self.work.data_ptr(), xb.data_ptr(),
+                                           vb.data_ptr(), Fb.data_ptr(), Cb.data_ptr(), mub.data_ptr(),
+                                           lamb.data_ptr(), self.oob.data_ptr()), "mpm_adjoint_step")
+
+    def check_oob(self):
+        n = int(self.oob.item())
+        if n:
+            self.oob.zero_()
+            raise RuntimeError(f"{n} particle stencil(s) left the grid")  # three_d/p2g.py:51-52
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x0, v0, F0, C0, mu0, lam0, mass, cfg: RolloutConfig):
+        dev = x0.device
+        n = x0.shape[0]
+        st = DenseStepper(cfg, n, dev)
+        mass_, mu_, lam_ = _f32(mass, dev), _f32(mu0, dev), _f32(lam0, dev)
+        x, v = _f32(x0, dev).clone(), _f32(v0, dev).clone()
+        F, C = _f32(F0, dev).clone(), _f32(C0, dev).clone()
+        K = max(1, int(cfg.checkpoint_every))
+        ckpts = []
+        for s in range(cfg.nsteps):
+            if s % K == 0:
+                ckpts.append((x.clone(), v.clone(), F.clone(), C.clone()))
+            st.step(x, v, F, C, mass_, mu_, lam_)
+        st.check_oob()
+        ctx.stepper, ctx.ckpts, ctx.cfg = st, ckpts, cfg
+        ctx.mats = (mass_, mu_, lam_)
+        ctx.shapes = (mu0.shape, lam0.shape, mu0.dtype, lam0.dtype)
+        return x, v
+
+    @staticmethod
+    def backward(ctx, gx, gv):
+        st, cfg = ctx.stepper, ctx.cfg
+        mass_, mu_, lam_ = ctx.mats
+        dev = st.device
+        n = st.n
+        xb = _f32(gx, dev).clone() if gx is not None else torch.zeros((n, 3), device=dev)
+        vb = _f32(gv, dev).clone() if gv is not None else torch.zeros((n, 3), device=dev)
+        Fb = torch.zeros((n, 3, 3), dtype=torch.float32, device=dev)
+        Cb = torch.zeros((n, 3, 3), dtype=torch.float32, device=dev)
+        mub = torch.zeros_like(mu_)
+        lamb = torch.zeros_like(lam_)
+        K = max(1, int(cfg.checkpoint_every))
+        for seg in range(len(ctx.ckpts) - 1, -1, -1):
+            s0 = seg * K
+            s1 = min(s0 + K, cfg.nsteps)
+            x, v, F, C = (t.clone() for t in ctx.ckpts[seg])
+            states = []
+            for _ in range(s0, s1):
+                states.append((x.clone(), v.clone(), F.clone(), C.clone()))
+                st.step(x, v, F, C, mass_, mu_, lam_)
+            for (x, v, F, C) in reversed(states):
+                st.adjoint(x, v, F, C, mass_, mu_, lam_, xb, vb, Fb, Cb, mub, lamb)
+            del states
+        st.check_oob()
+        mu_shape, lam_shape, mu_dt, lam_dt = ctx.shapes
+        ctx.ckpts = None
+        return (xb, vb, Fb, Cb, mub.reshape(mu_shape).to(mu_dt), lamb.reshape(lam_shape).to(lam_dt), None, None)
+
+
+def rollout(x0: torch.Tensor, v0: torch.Tensor, mu0: torch.Tensor, lam0: torch.Tensor, mass: torch.Tensor,
+            cfg: RolloutConfig, F0: Optional[torch.Tensor] = None,
+            C0: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, torch.Tensor]:
+    """Final (x, v) after cfg.nsteps substeps; differentiable in x0, v0, F0,
+    C0, mu0 and lam0.  Inputs are CUDA tensors: x0, v0 (n, 3); mu0, lam0,
+    mass (n,) or one element; F0, C0 (n, 3, 3), default identity / zero."""
+    n = x0.shape[0]
+    if F0 is None:
+        F0 = torch.eye(3, device=x0.device, dtype=torch.float32).expand(n, 3, 3)
+    if C0 is None:
+        C0 = torch.zeros((n, 3, 3), device=x0.device, dtype=torch.float32)
+    return _Rollout.apply(x0, v0, F0, C0, mu0, lam0, mass, cfg)
+
+
+def lame_from_E(E: torch.Tensor, nu: float):
+    """numerics/fem.py:1-6: mu = E / (2 (1 + nu)), lambda = E nu / ((1 + nu)(1 - 2 nu))."""
+    return E / (2.0 * (1.0 + nu)), E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))

@@ -83,6 +83,56 @@ int32_t mpm_p2g(void* stream, const mpm_params* prm, const mpm_particles* p, int
   return MPM_OK;
 }
 
+static int check_dense(const mpm_params* prm, const mpm_particles* p, const void* work, const char* fn) {
+  int rc;
+  if ((rc = check_params(prm, fn)) || (rc = check_particles(p, fn, true))) return rc;
+  if (prm->dim != 3 || p->dtype != MPM_F32 || !work) {
+    set_error(std::string(fn) + ": needs dim 3, FP32 particles and a workspace");
+    return MPM_ERR_ARG;
+  }
+  return MPM_OK;
+}
+
+int32_t mpm_dense_workspace_bytes(const mpm_params* prm, int64_t* bytes) {
+  int rc;
+  if ((rc = check_params(prm, "mpm_dense_workspace_bytes"))) return rc;
+  if (!bytes) {
+    set_error("mpm_dense_workspace_bytes: null output");
+    return MPM_ERR_ARG;
+  }
+  *bytes = (int64_t)(sizeof(float) * mpm::adjoint_workspace_floats(prm->res));
+  return MPM_OK;
+}
+
+int32_t mpm_dense_substep(void* stream, const mpm_params* prm, mpm_particles* p, double hardening,
+                          void* work, int64_t* oob_count) {
+  int rc;
+  if ((rc = check_dense(prm, p, work, "mpm_dense_substep"))) return rc;
+  const int64_t r1 = prm->res + 1, nn = r1 * r1 * r1;
+  float* gv = (float*)work + 4 * nn;
+  mpm::dense_forward_grids((hipStream_t)stream, prm, p, hardening, (float*)work,
+                           (unsigned long long*)oob_count);
+  mpm::phase_g2p((hipStream_t)stream, prm, p, MPM_ELASTIC, 0.0, 0.0, nullptr, 0, gv,
+                 (unsigned long long*)oob_count);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particles* p, double hardening,
+                         void* work, float* x_bar, float* v_bar, float* F_bar, float* C_bar,
+                         float* mu0_bar, float* lambda0_bar, int64_t* oob_count) {
+  int rc;
+  if ((rc = check_dense(prm, p, work, "mpm_adjoint_step"))) return rc;
+  if (p->n > 0 && (!x_bar || !v_bar || !F_bar || !C_bar || !mu0_bar || !lambda0_bar)) {
+    set_error("mpm_adjoint_step: null adjoint array");
+    return MPM_ERR_ARG;
+  }
+  mpm::adjoint_step((hipStream_t)stream, prm, p, hardening, (float*)work, x_bar, v_bar, F_bar, C_bar,
+                    mu0_bar, lambda0_bar, (unsigned long long*)oob_count);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
 int32_t mpm_grid_op(void* stream, const mpm_params* prm, int32_t dtype, void* grid_v,
                     const void* grid_m) {
   int rc;

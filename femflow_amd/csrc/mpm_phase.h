@@ -17,4 +17,11 @@ void phase_g2p(hipStream_t st, const mpm_params* prm, mpm_particles* p, int kind
                double theta_c, double theta_s, const mpm_material* mats, int nmat, const void* gv,
                unsigned long long* oob);
 
+size_t adjoint_workspace_floats(int res);
+void dense_forward_grids(hipStream_t st, const mpm_params* prm, const mpm_particles* p, double hardening,
+                         float* work, unsigned long long* oob);
+void adjoint_step(hipStream_t st, const mpm_params* prm, const mpm_particles* p, double hardening,
+                  float* work, float* xb, float* vb, float* Fb, float* Cb, float* mub, float* lamb,
+                  unsigned long long* oob);
+
 }  // namespace mpm

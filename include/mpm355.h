@@ -186,6 +186,29 @@ int32_t mpm_particle_sums(mpm_ctx* ctx, double* out);
 int32_t mpm_bin(mpm_ctx* ctx, int32_t dim, int32_t res, double inv_dx, int32_t layout,
                 const float* x, int64_t n, uint32_t* keys, uint32_t* perm);
 
+/* ---- differentiable rollouts: dense-grid substep and its adjoint ------
+ * Elastic (fixed-corotated, utils.py:95-135 with constant_hardening,
+ * utils.py:7-24) 3D substeps on a dense (res+1)^3 grid held in a caller
+ * workspace of mpm_dense_workspace_bytes() bytes (device, 16-byte aligned).
+ * FP32 particles (p->dtype == MPM_F32, per-particle or scalar mass/mu0/lambda0).
+ * No reference counterpart: the reference has no gradients.  The substep is
+ * the one mls_mpm.py:40-79 takes (three_d/p2g.py:14-80,
+ * three_d/grid_op.py:5-47, three_d/g2p.py:9-59) and is what the adjoint
+ * reverses. */
+int32_t mpm_dense_workspace_bytes(const mpm_params* prm, int64_t* bytes);
+/* One forward substep in place on p (x, v, F, C), using the workspace grids. */
+int32_t mpm_dense_substep(void* stream, const mpm_params* prm, mpm_particles* p, double hardening,
+                          void* work, int64_t* oob_count);
+/* Reverse of one substep taken from state p (unchanged).  On entry x_bar,
+ * v_bar, F_bar, C_bar (device float, n*3 / n*3 / n*9 / n*9) hold the adjoints
+ * of the substep's outputs; on return, the adjoints of its inputs.  mu0_bar
+ * and lambda0_bar (n floats, or 1 when p->mat_scalar) are accumulated into.
+ * The stencil base, the velocity clamp and the wall masks are differentiated
+ * piecewise (held fixed / no gradient where active). */
+int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particles* p, double hardening,
+                         void* work, float* x_bar, float* v_bar, float* F_bar, float* C_bar,
+                         float* mu0_bar, float* lambda0_bar, int64_t* oob_count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,238 @@
+"""Hand-written adjoint of the elastic substep (femflow_amd.adjoint).
+
+Reference gradient: torch.autograd through oracle/mpm_torch.py (FP64, CPU),
+itself pinned to the NumPy oracle's forward and to central differences.  The
+GPU adjoint is compared with it per substep and over a checkpointed rollout;
+the full-size rollout (2^17 particles, 64^3 grid, 500 substeps) is checked by
+directional central differences of the FP64 GPU forward (h = 1e-4).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mpm_oracle as O
+from oracle import mpm_torch as T
+
+
+def _cube_scene(n, res, seed, lo=0.35, side=0.25, vscale=0.3, fnoise=0.02, cnoise=0.5):
+    rng = np.random.default_rng(seed)
+    x = lo + side * rng.random((n, 3))
+    v = vscale * rng.standard_normal((n, 3))
+    F = np.eye(3)[None] + fnoise * rng.standard_normal((n, 3, 3))
+    C = cnoise * rng.standard_normal((n, 3, 3))
+    E = 1e3 + 9e3 * rng.random(n)
+    mu0, lam0 = O.Ev_to_mu(E, 0.2), O.Ev_to_lambda(E, 0.2)
+    vol = side ** 3 / n
+    mass = np.full(n, vol)
+    return dict(x=x, v=v, F=F, C=C, mu0=mu0, lam0=lam0, mass=mass, vol=vol, res=res, E=E)
+
+
+def _t64(a):
+    return torch.tensor(np.asarray(a), dtype=torch.float64)
+
+
+# ------------------------------------------------------------------ CPU ----
+@pytest.mark.parametrize("quirks", ["reference", "textbook"])
+def test_torch_reference_forward_matches_oracle(quirks):
+    sc = _cube_scene(300, 16, 1)
+    dt = 1e-4
+    out = T.substep(_t64(sc["x"]), _t64(sc["v"]), _t64(sc["F"]), _t64(sc["C"]), _t64(sc["mass"]),
+                    _t64(sc["mu0"]), _t64(sc["lam0"]), 16, dt, sc["vol"], hardening=1.3, quirks=quirks)
+    x, v, F, C = (sc[k].copy() for k in ("x", "v", "F", "C"))
+    Jp = np.ones((300, 1))
+    O.solve_mls_mpm_3d(16, 16.0, 1.3, 1 / 16, dt, sc["vol"], -9.8, x, sc["mass"], sc["mu0"], sc["lam0"],
+                       v, F, C, Jp, quirks=quirks)
+    for got, ref in zip(out, (x, v, F, C)):
+        g = got.detach().numpy()
+        assert np.abs(g - ref).max() <= 1e-11 * max(1.0, np.abs(ref).max())
+
+
+def test_torch_reference_gradient_matches_central_differences():
+    sc = _cube_scene(40, 8, 2, lo=0.3, side=0.35)
+    dt, steps = 2e-4, 3
+    rng = np.random.default_rng(7)
+    target = _t64(rng.random(3))
+    dv, dmu = _t64(rng.standard_normal((40, 3))), _t64(rng.standard_normal(40))
+
+    def loss(v0, mu0):
+        x, v, F, C = T.rollout(_t64(sc["x"]), v0, _t64(sc["F"]), _t64(sc["C"]), _t64(sc["mass"]), mu0,
+                               _t64(sc["lam0"]), 8, dt, sc["vol"], steps)
+        return ((x.mean(0) - target) ** 2).sum() + 1e-3 * (v ** 2).sum()
+
+    v0 = _t64(sc["v"]).requires_grad_(True)
+    mu0 = _t64(sc["mu0"]).requires_grad_(True)
+    L = loss(v0, mu0)
+    gv, gmu = torch.autograd.grad(L, (v0, mu0))
+    with torch.no_grad():
+        h = 1e-6
+        fd_v = (loss(v0 + h * dv, mu0) - loss(v0 - h * dv, mu0)) / (2 * h)
+        hm = 1e-3
+        fd_m = (loss(v0, mu0 + hm * dmu) - loss(v0, mu0 - hm * dmu)) / (2 * hm)
+    assert abs(float((gv * dv).sum()) - float(fd_v)) <= 1e-7 * abs(float(fd_v)) + 1e-12
+    assert abs(float((gmu * dmu).sum()) - float(fd_m)) <= 1e-5 * abs(float(fd_m)) + 1e-14
+
+
+def test_polar_newton_matches_svd_polar():
+    rng = np.random.default_rng(3)
+    F = np.eye(3)[None] + 0.2 * rng.standard_normal((50, 3, 3))
+    R = T.polar_newton(_t64(F)).numpy()
+    assert np.abs(R - O.polar_decomp_3d(F)).max() < 1e-12
+
+
+# ------------------------------------------------------------------ GPU ----
+def _rel(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("quirks", ["reference", "textbook"])
+def test_adjoint_step_matches_fp64_autograd(gpu, quirks):
+    from femflow_amd.adjoint import DenseStepper, RolloutConfig
+
+    n, res, dt = 512, 16, 1e-4
+    sc = _cube_scene(n, res, 11)
+    cfg = RolloutConfig(res=res, dt=dt, volume=sc["vol"], nsteps=1, hardening=1.2, quirks=quirks)
+    st = DenseStepper(cfg, n, gpu)
+    f32 = lambda a: torch.tensor(np.asarray(a), dtype=torch.float32, device=gpu).contiguous()  # noqa: E731
+    x, v, F, C = f32(sc["x"]), f32(sc["v"]), f32(sc["F"]), f32(sc["C"])
+    mass, mu0, lam0 = f32(sc["mass"]), f32(sc["mu0"]), f32(sc["lam0"])
+    rng = np.random.default_rng(5)
+    ax, av, aF, aC = (rng.standard_normal(s) for s in ((n, 3), (n, 3), (n, 3, 3), (n, 3, 3)))
+    xb, vb, Fb, Cb = f32(ax), f32(av), f32(aF), f32(aC)
+    mub, lamb = torch.zeros_like(mu0), torch.zeros_like(lam0)
+    st.adjoint(x, v, F, C, mass, mu0, lam0, xb, vb, Fb, Cb, mub, lamb)
+    torch.cuda.synchronize()
+    st.check_oob()
+    # FP64 reference on the same FP32 inputs
+    ins = [t.detach().double().cpu().requires_grad_(True) for t in (x, v, F, C, mu0, lam0)]
+    out = T.substep(ins[0], ins[1], ins[2], ins[3], mass.double().cpu(), ins[4], ins[5], res, dt,
+                    sc["vol"], hardening=1.2, quirks=quirks)
+    L = sum((o * _t64(a)).sum() for o, a in zip(out, (ax, av, aF, aC)))
+    ref = torch.autograd.grad(L, ins)
+    got = [xb, vb, Fb, Cb, mub, lamb]
+    for name, g, r in zip(("x", "v", "F", "C", "mu0", "lam0"), got, ref):
+        e = _rel(g.cpu().numpy(), r.numpy())
+        print(f"adjoint step {quirks} d/d{name}: rel err {e:.2e}", flush=True)
+        assert e < 2e-4, name
+
+
+@pytest.mark.gpu
+def test_rollout_gradient_matches_fp64_reference(gpu):
+    from femflow_amd.adjoint import RolloutConfig, rollout
+
+    n, res, dt, steps = 1024, 32, 1e-4, 120
+    sc = _cube_scene(n, res, 21, vscale=0.5, fnoise=0.0, cnoise=0.0)
+    F0 = np.repeat(np.eye(3)[None], n, 0)
+    C0 = np.zeros((n, 3, 3))
+    cfg = RolloutConfig(res=res, dt=dt, volume=sc["vol"], nsteps=steps, checkpoint_every=50)
+    rng = np.random.default_rng(0)
+    target = sc["x"].mean(0) + 0.02 * rng.standard_normal(3)
+    f32 = lambda a: torch.tensor(np.asarray(a), dtype=torch.float32, device=gpu)  # noqa: E731
+    x0, v0 = f32(sc["x"]).requires_grad_(True), f32(sc["v"]).requires_grad_(True)
+    mu0, lam0 = f32(sc["mu0"]).requires_grad_(True), f32(sc["lam0"]).requires_grad_(True)
+    xT, vT = rollout(x0, v0, mu0, lam0, f32(sc["mass"]), cfg)
+    L = ((xT.mean(0) - f32(target)) ** 2).sum() + 1e-3 * (vT ** 2).mean()
+    gx, gv, gmu, glam = torch.autograd.grad(L, (x0, v0, mu0, lam0))
+    # FP64 autograd on the CPU, from the same FP32 starting values
+    r = [t.detach().double().cpu().requires_grad_(True) for t in (x0, v0, mu0, lam0)]
+    xr, vr, _, _ = T.rollout(r[0], r[1], _t64(F0), _t64(C0), _t64(sc["mass"]).float().double(), r[2], r[3],
+                             res, dt, sc["vol"], steps)
+    Lr = ((xr.mean(0) - _t64(target).float().double()) ** 2).sum() + 1e-3 * (vr ** 2).mean()
+    ref = torch.autograd.grad(Lr, r)
+    print("rollout loss gpu", float(L), "fp64", float(Lr), flush=True)
+    for name, g, rr in zip(("x0", "v0", "mu0", "lam0"), (gx, gv, gmu, glam), ref):
+        e = _rel(g.cpu().numpy(), rr.numpy())
+        print(f"rollout d/d{name}: rel err {e:.2e}", flush=True)
+        assert e < 5e-3, name
+
+
+def _fp64_gpu_loss(gpu, x0, v0, mu0, lam0, mass, vol, res, dt, steps, sub, target):
+    """FP64 dense-grid forward on the GPU (the library's FP64 phase kernels)."""
+    from femflow_amd import _lib
+    from femflow_amd.solvers.mpm import _phase
+
+    n = x0.shape[0]
+    f64 = lambda a: torch.as_tensor(a, dtype=torch.float64, device=gpu).clone().contiguous()  # noqa: E731
+    x, v = f64(x0), f64(v0)
+    F = torch.eye(3, dtype=torch.float64, device=gpu).repeat(n, 1, 1)
+    C = torch.zeros((n, 3, 3), dtype=torch.float64, device=gpu)
+    Jp = torch.ones(n, dtype=torch.float64, device=gpu)
+    sp = _phase.StagedParticles(x, v, F, C, Jp, f64(mass), f64(mu0), f64(lam0))
+    prm = _phase.make_params(3, res, float(res), 1.0 / res, dt, vol, -9.8)
+    r1 = res + 1
+    gv = torch.zeros((r1, r1, r1, 3), dtype=torch.float64, device=gpu)
+    gm = torch.zeros((r1, r1, r1, 1), dtype=torch.float64, device=gpu)
+    for _ in range(steps):
+        gv.zero_()
+        gm.zero_()
+        assert _phase.run_p2g(sp, prm, _lib.MPM_ELASTIC, 1.0, gv, gm) == 0
+        _phase.run_grid_op(prm, gv, gm)
+        assert _phase.run_g2p(sp, prm, _lib.MPM_ELASTIC, gv) == 0
+    c = x[torch.as_tensor(sub, device=gpu)].mean(0)
+    return float(((c - torch.as_tensor(target, dtype=torch.float64, device=gpu)) ** 2).sum())
+
+
+def full_size_adjoint_case(seed=0):
+    """The full-size gradient workload: 2^17 particles uniform in the cube
+    [0.4, 0.6]^3 on a 64^3 grid, E ~ U[1e3, 1e4] per particle (nu = 0.2),
+    density 1, dt = 1e-4, 500 substeps; v0 = one seeded drift plus a
+    compression along y (v_y -= 3 (y - 0.5)); loss = |centroid of the upper
+    half (y0 > 0.5) at T - target|^2.  (The whole-body centroid would not do:
+    internal forces conserve momentum, so its gradient in E is identically
+    zero while no wall is reached.)"""
+    n, res, dt, steps = 1 << 17, 64, 1e-4, 500
+    rng = np.random.default_rng(seed)
+    x0 = 0.4 + 0.2 * rng.random((n, 3))
+    v0 = np.tile(0.5 * rng.standard_normal(3), (n, 1))
+    v0[:, 1] -= 3.0 * (x0[:, 1] - 0.5)
+    E = 1e3 + 9e3 * rng.random(n)
+    sub = np.nonzero(x0[:, 1] > 0.5)[0]
+    target = x0[sub].mean(0) + 0.05 * rng.standard_normal(3)
+    vol = 0.2 ** 3 / n
+    return dict(n=n, res=res, dt=dt, steps=steps, x0=x0, v0=v0, E=E, sub=sub, target=target, vol=vol,
+                mass=np.full(n, vol), rng=rng)
+
+
+@pytest.mark.gpu
+def test_rollout_full_size_matches_central_differences(gpu):
+    import time
+
+    from femflow_amd.adjoint import RolloutConfig, lame_from_E, rollout
+
+    c = full_size_adjoint_case()
+    n, res, dt, steps, vol, rng = c["n"], c["res"], c["dt"], c["steps"], c["vol"], c["rng"]
+    cfg = RolloutConfig(res=res, dt=dt, volume=vol, nsteps=steps, checkpoint_every=50)
+    f32 = lambda a: torch.tensor(np.asarray(a), dtype=torch.float32, device=gpu)  # noqa: E731
+    v0t, Et = f32(c["v0"]).requires_grad_(True), f32(c["E"]).requires_grad_(True)
+    sub = torch.as_tensor(c["sub"], device=gpu)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mu0, lam0 = lame_from_E(Et, 0.2)
+    xT, _ = rollout(f32(c["x0"]), v0t, mu0, lam0, f32(c["mass"]), cfg)
+    L = ((xT[sub].mean(0) - f32(c["target"])) ** 2).sum()
+    gv, gE = torch.autograd.grad(L, (v0t, Et))
+    torch.cuda.synchronize()
+    print(f"full-size rollout forward+backward: {time.perf_counter() - t0:.2f} s, loss {float(L):.6e}", flush=True)
+    # directional central differences of the FP64 forward (same FP32 inputs), h = 1e-4
+    x0_, v0_ = f32(c["x0"]).double().cpu().numpy(), v0t.detach().double().cpu().numpy()
+    E_, m_ = Et.detach().double().cpu().numpy(), f32(c["mass"]).double().cpu().numpy()
+    tgt = f32(c["target"]).double().cpu().numpy()
+    h = 1e-4
+
+    def loss64(v0, E):
+        mu, lam = lame_from_E(torch.as_tensor(E), 0.2)
+        return _fp64_gpu_loss(gpu, x0_, v0, mu, lam, m_, vol, res, dt, steps, c["sub"], tgt)
+
+    d = rng.standard_normal((n, 3))
+    d /= np.linalg.norm(d)
+    fd = (loss64(v0_ + h * d, E_) - loss64(v0_ - h * d, E_)) / (2 * h)
+    ad = float((gv.double().cpu().numpy() * d).sum())
+    print(f"dL/dv0 . d: adjoint {ad:.6e} fd {fd:.6e}", flush=True)
+    assert abs(ad - fd) <= 2e-2 * abs(fd)
+    for name, dE in (("uniform", E_.copy()), ("random", E_ * rng.standard_normal(n))):
+        fd = (loss64(v0_, E_ + h * dE) - loss64(v0_, E_ - h * dE)) / (2 * h)
+        ad = float((gE.double().cpu().numpy() * dE).sum())
+        print(f"dL/dE . d[{name}]: adjoint {ad:.6e} fd {fd:.6e}", flush=True)
+        assert abs(ad - fd) <= 5e-2 * abs(fd), name
