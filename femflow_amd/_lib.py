@@ -27,6 +27,7 @@ EXPORTS = (
     "mpm_ctx_kernel_timing", "mpm_ctx_kernel_times", "mpm_state_upload", "mpm_state_download",
     "mpm_run", "mpm_run_frames", "mpm_state_positions", "mpm_substep", "mpm_ctx_oob_count", "mpm_ctx_fallback_count",
     "mpm_particle_sums", "mpm_bin", "mpm_dense_workspace_bytes", "mpm_dense_substep", "mpm_adjoint_step",
+    "mpm_mlp_sizes", "mpm_mlp_pack", "mpm_mlp_forward", "mpm_mlp_loss", "mpm_mlp_backward", "mpm_mlp_adam",
 )
 
 
@@ -104,6 +105,12 @@ def load():
             "mpm_dense_workspace_bytes": ([Q, ct.POINTER(i64)], i32),
             "mpm_dense_substep": ([vp, Q, P, dbl, vp, vp], i32),
             "mpm_adjoint_step": ([vp, Q, P, dbl, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+            "mpm_mlp_sizes": ([ct.POINTER(i64), ct.POINTER(i64)], i32),
+            "mpm_mlp_pack": ([vp, vp, vp], i32),
+            "mpm_mlp_forward": ([vp, vp, vp, vp, i64, vp, vp, vp], i32),
+            "mpm_mlp_loss": ([vp, vp, vp, vp, i64, ct.c_float, vp, vp], i32),
+            "mpm_mlp_backward": ([vp, vp, vp, i64, vp, vp, vp, vp, vp, vp], i32),
+            "mpm_mlp_adam": ([vp, vp, vp, vp, vp, i64, ct.c_float, ct.c_float, ct.c_float, ct.c_float, i32], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

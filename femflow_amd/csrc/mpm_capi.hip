@@ -5,6 +5,7 @@
 
 #include "mpm355.h"
 #include "mpm_engine.h"
+#include "mpm_mlp.h"
 #include "mpm_phase.h"
 #include "mpm_sort.h"
 
@@ -129,6 +130,71 @@ int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particle
   }
   mpm::adjoint_step((hipStream_t)stream, prm, p, hardening, (float*)work, x_bar, v_bar, F_bar, C_bar,
                     mu0_bar, lambda0_bar, (unsigned long long*)oob_count);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int32_t mpm_mlp_sizes(int64_t* n_params, int64_t* n_packed) {
+  if (!n_params || !n_packed) {
+    set_error("mpm_mlp_sizes: null output");
+    return MPM_ERR_ARG;
+  }
+  *n_params = mpm::P_COUNT;
+  *n_packed = mpm::K_COUNT;
+  return MPM_OK;
+}
+
+int32_t mpm_mlp_pack(void* stream, const float* params, uint16_t* packed) {
+  if (!params || !packed) {
+    set_error("mpm_mlp_pack: null buffer");
+    return MPM_ERR_ARG;
+  }
+  mpm::mlp_pack((hipStream_t)stream, params, packed);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int32_t mpm_mlp_forward(void* stream, const uint16_t* packed, const float* params, const uint16_t* X,
+                        int64_t rows, uint16_t* H1, uint16_t* H2, float* Y) {
+  if (rows < 0 || (rows > 0 && (!packed || !params || !X || !H1 || !H2 || !Y))) {
+    set_error("mpm_mlp_forward: bad argument");
+    return MPM_ERR_ARG;
+  }
+  mpm::mlp_forward((hipStream_t)stream, packed, params, X, rows, H1, H2, Y);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int32_t mpm_mlp_loss(void* stream, const float* Y, const float* T, const float* w, int64_t rows,
+                     float inv_norm, uint16_t* dY, double* loss) {
+  if (rows < 0 || (rows > 0 && (!Y || !T || !dY || !loss))) {
+    set_error("mpm_mlp_loss: bad argument");
+    return MPM_ERR_ARG;
+  }
+  mpm::mlp_loss((hipStream_t)stream, Y, T, w, rows, inv_norm, dY, loss);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int32_t mpm_mlp_backward(void* stream, const uint16_t* packed, const uint16_t* X, int64_t rows,
+                         const uint16_t* H1, const uint16_t* H2, const uint16_t* dY, uint16_t* dH2,
+                         uint16_t* dH1, float* grads) {
+  if (rows < 0 || (rows > 0 && (!packed || !X || !H1 || !H2 || !dY || !dH2 || !dH1 || !grads))) {
+    set_error("mpm_mlp_backward: bad argument");
+    return MPM_ERR_ARG;
+  }
+  mpm::mlp_backward((hipStream_t)stream, packed, X, rows, H1, H2, dY, dH2, dH1, grads);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int32_t mpm_mlp_adam(void* stream, float* params, const float* grads, float* m, float* v, int64_t n,
+                     float lr, float beta1, float beta2, float eps, int32_t step) {
+  if (!params || !grads || !m || !v || n <= 0 || n > (int64_t)1 << 30 || step < 1) {
+    set_error("mpm_mlp_adam: bad argument");
+    return MPM_ERR_ARG;
+  }
+  mpm::mlp_adam((hipStream_t)stream, params, grads, m, v, (int)n, lr, beta1, beta2, eps, step);
   CHECK_HIP(hipGetLastError());
   return MPM_OK;
 }

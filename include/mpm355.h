@@ -209,6 +209,30 @@ int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particle
                          void* work, float* x_bar, float* v_bar, float* F_bar, float* C_bar,
                          float* mu0_bar, float* lambda0_bar, int64_t* oob_count);
 
+/* ---- grid-corrector MLP (BF16 MFMA, FP32 accumulate) -------------------
+ * 351 node-stencil features -> 128 -> 128 -> 3, ReLU hidden layers.  No
+ * reference counterpart (the reference has no learned component).  Device
+ * buffers, row-major: X rows x 352 BF16 (column 351 zero), H1/H2/dH1/dH2
+ * rows x 128 BF16, Y rows x 16 FP32 (3 used), targets T rows x 3 FP32,
+ * dY rows x 32 BF16.  params: FP32 master copy (n_params floats: W1 [128][351],
+ * b1, W2 [128][128], b2, W3 [3][128], b3); packed: BF16 operand copies
+ * (n_packed uint16) made by mpm_mlp_pack. */
+int32_t mpm_mlp_sizes(int64_t* n_params, int64_t* n_packed);
+int32_t mpm_mlp_pack(void* stream, const float* params, uint16_t* packed);
+/* Y = MLP(X); H1, H2 receive the hidden activations (needed by backward). */
+int32_t mpm_mlp_forward(void* stream, const uint16_t* packed, const float* params, const uint16_t* X,
+                        int64_t rows, uint16_t* H1, uint16_t* H2, float* Y);
+/* loss += inv_norm * sum_i w_i |Y_i - T_i|^2 (w NULL = 1); dY = dloss/dY. */
+int32_t mpm_mlp_loss(void* stream, const float* Y, const float* T, const float* w, int64_t rows,
+                     float inv_norm, uint16_t* dY, double* loss);
+/* grads (n_params floats, accumulated) of the loss whose dY is given. */
+int32_t mpm_mlp_backward(void* stream, const uint16_t* packed, const uint16_t* X, int64_t rows,
+                         const uint16_t* H1, const uint16_t* H2, const uint16_t* dY, uint16_t* dH2,
+                         uint16_t* dH1, float* grads);
+/* One Adam step (FP32) on n parameters; step counts from 1. */
+int32_t mpm_mlp_adam(void* stream, float* params, const float* grads, float* m, float* v, int64_t n,
+                     float lr, float beta1, float beta2, float eps, int32_t step);
+
 #ifdef __cplusplus
 }
 #endif
