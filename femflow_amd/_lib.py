@@ -27,6 +27,7 @@ EXPORTS = (
     "mpm_ctx_kernel_timing", "mpm_ctx_kernel_times", "mpm_state_upload", "mpm_state_download",
     "mpm_run", "mpm_run_frames", "mpm_state_positions", "mpm_substep", "mpm_ctx_oob_count", "mpm_ctx_fallback_count",
     "mpm_particle_sums", "mpm_bin", "mpm_dense_workspace_bytes", "mpm_dense_substep", "mpm_adjoint_step",
+    "mpm_ctx_set_corrector", "mpm_corrector_sample", "mpm_grid_velocity",
     "mpm_mlp_sizes", "mpm_mlp_pack", "mpm_mlp_forward", "mpm_mlp_loss", "mpm_mlp_backward", "mpm_mlp_adam",
 )
 
@@ -105,6 +106,9 @@ def load():
             "mpm_dense_workspace_bytes": ([Q, ct.POINTER(i64)], i32),
             "mpm_dense_substep": ([vp, Q, P, dbl, vp, vp], i32),
             "mpm_adjoint_step": ([vp, Q, P, dbl, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+            "mpm_ctx_set_corrector": ([vp, vp, ct.c_float, ct.c_float, i32], i32),
+            "mpm_corrector_sample": ([vp, vp, vp, vp, i64, ct.POINTER(i64)], i32),
+            "mpm_grid_velocity": ([vp, vp, i64, vp], i32),
             "mpm_mlp_sizes": ([ct.POINTER(i64), ct.POINTER(i64)], i32),
             "mpm_mlp_pack": ([vp, vp, vp], i32),
             "mpm_mlp_forward": ([vp, vp, vp, vp, i64, vp, vp, vp], i32),

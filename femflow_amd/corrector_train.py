@@ -1,0 +1,302 @@
+"""Training the learned corrector on the solver's own rollouts, and running
+corrected rollouts with the plain substep as fallback.
+
+Training data.  A rollout of the three-cube scene (three materials, seeded
+cube velocities) runs plain substeps at dt.  At sampled times t (multiples
+of factor = dt_big / dt) its state s(t) gives one training sample:
+  * features and the big step's updated grid velocity v_big: one P2G + grid
+    update of s(t) at dt_big (engine sampling mode);
+  * target grid velocity v*: the P2G (mass-weighted average) of each
+    particle's mean velocity over the big step, (x(t + dt_big) - x(t)) /
+    dt_big, from `factor` plain substeps, with the wall conditions applied;
+  * target correction (v* - v_big) / v_ref, weighted by node mass.
+A corrected big step then replaces `factor` plain substeps: P2G + grid update
+at dt_big, plus the network's correction, then G2P.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+from typing import Callable, Dict, List, Optional
+
+import numpy as np
+import torch
+
+from .corrector import CorrectorMLP
+from .engine import Engine, MaterialSpec, SceneSpec
+from .scenes import MATERIALS_C2, scene_3d_cubes
+
+MATERIAL_TABLE = [MaterialSpec(("elastic", "snow", "water")[k], h, tc, ts) for k, _, _, h, tc, ts in MATERIALS_C2]
+CENTER = 13 * 13  # column of the node's own mass feature (neighbour (0,0,0))
+
+
+@dataclass
+class CorrectorConfig:
+    n: int = 1 << 20
+    res: int = 128
+    dt: float = 5e-5
+    factor: int = 8
+    gravity: float = -9.8
+    v_ref: float = 1.0
+    rho: float = 1.0
+
+    @property
+    def dt_big(self):
+        return self.dt * self.factor
+
+    @property
+    def m_ref(self):
+        return self.rho / float(self.res) ** 3  # mass of a full node cell
+
+
+def scene_tensors(cfg: CorrectorConfig, seed: int, device) -> Dict[str, torch.Tensor]:
+    st, _ = scene_3d_cubes(n=cfg.n, res=cfg.res, seed=seed, dt=cfg.dt, gravity=cfg.gravity, rho=cfg.rho)
+    n = len(st["x"])
+    f = lambda a, shape=None: torch.tensor(np.ascontiguousarray(a), dtype=torch.float32,  # noqa: E731
+                                           device=device).reshape(shape if shape else np.shape(a))
+    return dict(x=f(st["x"]), v=f(st["v"]), F=f(st["F"], (n, 3, 3)), C=f(st["C"], (n, 3, 3)),
+                Jp=f(st["Jp"], (n,)), mass=f(st["mass"]), mu0=f(st["mu0"]), lam0=f(st["lam0"]),
+                kind=torch.tensor(st["kind"], dtype=torch.int32, device=device),
+                volume=float(st["mass"][0] / cfg.rho))
+
+
+def make_engine(cfg: CorrectorConfig, volume: float, dt: float, gravity: float, device, n: int,
+                chunk: int = 512, sort_every: int = 16) -> Engine:
+    eng = Engine([SceneSpec(res=cfg.res, dt=dt, gravity=gravity, volume=volume)], MATERIAL_TABLE, n_max=n,
+                 device=device)
+    eng.tune(chunk=chunk, sort_every=sort_every, grid_mult=8)
+    return eng
+
+
+def _upload(eng: Engine, s: Dict[str, torch.Tensor], **over):
+    g = lambda k: over.get(k, s[k])  # noqa: E731
+    eng.upload(g("x"), g("v"), g("F"), g("C"), g("Jp"), s["mass"], s["mu0"], s["lam0"], s["kind"])
+
+
+def _download(eng: Engine, s: Dict[str, torch.Tensor]) -> Dict[str, torch.Tensor]:
+    out = dict(s)
+    for k in ("x", "v", "F", "C", "Jp"):
+        out[k] = torch.empty_like(s[k])
+    eng.download(out["x"], out["v"], out["F"], out["C"], out["Jp"])
+    return out
+
+
+class SampleGenerator:
+    """Engines for the plain rollout (dt), the big-step features (dt_big) and
+    the target P2G (dt_big, no gravity, zero stress)."""
+
+    def __init__(self, cfg: CorrectorConfig, device):
+        self.cfg = cfg
+        self.device = torch.device(device)
+        s = scene_tensors(cfg, 0, device)
+        self.n = s["x"].shape[0]
+        vol = s["volume"]
+        self.volume = vol
+        self.small = make_engine(cfg, vol, cfg.dt, cfg.gravity, device, self.n)
+        self.big = make_engine(cfg, vol, cfg.dt_big, cfg.gravity, device, self.n, sort_every=1)
+        self.tgt = make_engine(cfg, vol, cfg.dt_big, 0.0, device, self.n, sort_every=1)
+        dummy = torch.zeros(CorrectorMLP.init_params().size, dtype=torch.float32, device=device)
+        self.big.set_corrector(dummy, cfg.m_ref, cfg.v_ref, enable=False)
+        self.tgt.set_corrector(dummy, cfg.m_ref, cfg.v_ref, enable=False)
+        self.eye = torch.eye(3, device=device).expand(self.n, 3, 3).contiguous()
+        self.zero33 = torch.zeros((self.n, 3, 3), device=device)
+        self.ones = torch.ones(self.n, device=device)
+        self.max_rows = min(2 * self.n, (cfg.res + 1) ** 3, 1 << 19)
+
+    def sample(self, s: Dict[str, torch.Tensor], rows: int, gen: torch.Generator):
+        """Training rows from state s (caller order); advances self.small by
+        `factor` substeps from s.  Returns X, T, w (node mass / m_ref)."""
+        cfg = self.cfg
+        _upload(self.big, s)
+        X, nodes, vbig, total = self.big.corrector_sample(max_rows=self.max_rows)
+        if total > self.max_rows:
+            raise RuntimeError(f"{total} nodes with mass exceed the sampling buffer ({self.max_rows})")
+        _upload(self.small, s)
+        self.small.run(cfg.factor)
+        x8 = self.small.positions()
+        u = (x8 - s["x"]) / cfg.dt_big
+        _upload(self.tgt, s, v=u.contiguous(), F=self.eye, C=self.zero33, Jp=self.ones)
+        self.tgt.p2g_grid_only()
+        vstar = self.tgt.grid_velocity(nodes)
+        T = (vstar - vbig) / cfg.v_ref
+        w = X[:, CENTER].float()
+        if X.shape[0] > rows:
+            idx = torch.randperm(X.shape[0], generator=gen, device="cpu")[:rows].to(self.device)
+            X, T, w = X[idx], T[idx], w[idx]
+        return X.contiguous(), T.contiguous(), w.contiguous()
+
+    def rollouts(self, n_rollouts: int, samples_per_rollout: int, steps: int, rows: int, seed: int = 0,
+                 log: Optional[Callable[[str], None]] = None):
+        """Rows from n_rollouts seeded rollouts of `steps` plain substeps."""
+        cfg = self.cfg
+        rng = np.random.default_rng(seed)
+        gen = torch.Generator().manual_seed(seed)
+        Xs, Ts, ws = [], [], []
+        t0 = time.perf_counter()
+        for r in range(n_rollouts):
+            s = scene_tensors(cfg, seed * 100003 + r + 1, self.device)
+            _upload(self.small, s)
+            slots = np.arange(0, steps - cfg.factor + 1, cfg.factor)
+            times = np.sort(rng.choice(slots, size=min(samples_per_rollout, len(slots)), replace=False))
+            t = 0
+            cur = s
+            for ts in times:
+                if ts > t:
+                    _upload(self.small, cur)
+                    self.small.run(int(ts - t))
+                    cur = _download(self.small, cur)
+                    t = int(ts)
+                X, T, w = self.sample(cur, rows, gen)
+                Xs.append(X)
+                Ts.append(T)
+                ws.append(w)
+                cur = _download(self.small, cur)  # the state factor substeps later
+                t += cfg.factor
+            if log and (r + 1) % max(1, n_rollouts // 10) == 0:
+                log(f"rollout {r + 1}/{n_rollouts}: {sum(x.shape[0] for x in Xs)} rows, "
+                    f"{time.perf_counter() - t0:.1f} s")
+        return torch.cat(Xs), torch.cat(Ts), torch.cat(ws)
+
+    def policy_rollouts(self, net: CorrectorMLP, n_rollouts: int, samples_per_rollout: int, n_big: int,
+                        rows: int, seed: int = 0, log: Optional[Callable[[str], None]] = None):
+        """Rows from states visited by corrected rollouts (the network's own
+        state distribution), with the same plain-substep targets."""
+        cfg = self.cfg
+        rng = np.random.default_rng(seed + 7)
+        gen = torch.Generator().manual_seed(seed + 7)
+        corr = make_engine(cfg, self.volume, cfg.dt_big, cfg.gravity, self.device, self.n)
+        corr.set_corrector(net.params, cfg.m_ref, cfg.v_ref, enable=True)
+        Xs, Ts, ws = [], [], []
+        vmax = 0.9 * (1.0 / cfg.res) / cfg.dt_big
+        try:
+            for r in range(n_rollouts):
+                s = scene_tensors(cfg, seed * 100003 + 50000 + r, self.device)
+                _upload(corr, s)
+                times = np.sort(rng.choice(np.arange(1, n_big + 1), size=samples_per_rollout, replace=False))
+                t = 0
+                cur = s
+                for tb in times:
+                    try:
+                        corr.run(int(tb - t), check_oob=True)
+                    except RuntimeError:
+                        break
+                    t = int(tb)
+                    cur = _download(corr, cur)
+                    if not bool(torch.isfinite(cur["x"]).all()) or float(cur["v"].norm(dim=1).max()) >= vmax:
+                        break
+                    X, T, w = self.sample(cur, rows, gen)
+                    Xs.append(X)
+                    Ts.append(T)
+                    ws.append(w)
+                if log and (r + 1) % max(1, n_rollouts // 5) == 0:
+                    log(f"policy rollout {r + 1}/{n_rollouts}: {sum(x.shape[0] for x in Xs)} rows")
+        finally:
+            corr.close()
+        if not Xs:
+            return None
+        return torch.cat(Xs), torch.cat(Ts), torch.cat(ws)
+
+    def close(self):
+        for e in (self.small, self.big, self.tgt):
+            e.close()
+
+
+def train(net: CorrectorMLP, X, T, w, epochs: int = 10, batch: int = 32768, lr: float = 1e-3,
+          val_frac: float = 0.1, seed: int = 0, log: Optional[Callable[[str], None]] = None) -> Dict:
+    """Adam on mini-batches with a cosine learning-rate decay; returns the
+    train/validation loss history (weighted MSE of the scaled correction)."""
+    n = X.shape[0]
+    nval = int(n * val_frac)
+    gen = torch.Generator().manual_seed(seed)
+    perm = torch.randperm(n, generator=gen).to(X.device)
+    val, tr = perm[:nval], perm[nval:]
+    Xv, Tv, wv = X[val].contiguous(), T[val].contiguous(), w[val].contiguous()
+    steps_per_epoch = max(1, len(tr) // batch)
+    total = epochs * steps_per_epoch
+    hist = {"train": [], "val": []}
+    step = 0
+    for ep in range(epochs):
+        order = tr[torch.randperm(len(tr), generator=gen).to(X.device)]
+        acc = torch.zeros(1, dtype=torch.float64, device=X.device)
+        for b in range(steps_per_epoch):
+            idx = order[b * batch:(b + 1) * batch]
+            cur_lr = lr * 0.5 * (1 + math.cos(math.pi * step / total))
+            acc += net.train_step(X[idx].contiguous(), T[idx].contiguous(), w[idx].contiguous(), lr=cur_lr)
+            step += 1
+        trl = float(acc.item()) / steps_per_epoch
+        vl = float(net.loss_grads(Xv, Tv, wv).item()) if nval else float("nan")
+        hist["train"].append(trl)
+        hist["val"].append(vl)
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: train {trl:.4e} val {vl:.4e}")
+    return hist
+
+
+def baseline_loss(T, w) -> float:
+    """Weighted MSE of predicting zero correction (the untrained reference)."""
+    return float(((T ** 2).sum(1) * w).sum() / w.sum())
+
+
+class CorrectedRunner:
+    """Corrected big steps with the plain substep as fallback.
+
+    Runs in chunks of `check_every` big steps.  After each chunk the state is
+    judged healthy when no stencil left the grid, every position is finite and
+    the largest particle speed stays below the grid's velocity clamp; an
+    unhealthy chunk is discarded and rerun with factor * check_every plain
+    substeps from its saved start state."""
+
+    def __init__(self, cfg: CorrectorConfig, net: Optional[CorrectorMLP], volume: float, device,
+                 check_every: int = 25):
+        self.cfg = cfg
+        self.device = torch.device(device)
+        self.net = net
+        self.volume = volume
+        self.check_every = check_every
+        self.fallbacks = 0
+
+    def run(self, s: Dict[str, torch.Tensor], n_big: int, judge: bool = True) -> Dict[str, torch.Tensor]:
+        cfg = self.cfg
+        n = s["x"].shape[0]
+        big = make_engine(cfg, self.volume, cfg.dt_big, cfg.gravity, self.device, n)
+        plain = make_engine(cfg, self.volume, cfg.dt, cfg.gravity, self.device, n) if judge else None
+        if self.net is not None:
+            big.set_corrector(self.net.params, cfg.m_ref, cfg.v_ref, enable=True)
+        cur = s
+        done = 0
+        vmax = 0.9 * (1.0 / cfg.res) / cfg.dt_big
+        try:
+            _upload(big, cur)
+            while done < n_big:
+                k = min(self.check_every, n_big - done)
+                if not judge:
+                    big.run(k, check_oob=False)
+                    done += k
+                    continue
+                start = _download(big, cur)
+                ok = True
+                try:
+                    big.run(k, check_oob=True)
+                except RuntimeError:
+                    ok = False
+                if ok:
+                    cur = _download(big, start)
+                    speed = cur["v"].norm(dim=1)
+                    ok = bool(torch.isfinite(cur["x"]).all()) and float(speed.max()) < vmax
+                if not ok:
+                    self.fallbacks += 1
+                    _upload(plain, start)
+                    plain.run(k * cfg.factor)
+                    cur = _download(plain, start)
+                    _upload(big, cur)
+                done += k
+            return _download(big, cur)
+        finally:
+            big.close()
+            if plain is not None:
+                plain.close()
+
+
+def rel_position_error(x, ref) -> float:
+    return float((x - ref).norm() / ref.norm())

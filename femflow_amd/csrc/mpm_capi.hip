@@ -134,6 +134,32 @@ int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particle
   return MPM_OK;
 }
 
+int32_t mpm_ctx_set_corrector(mpm_ctx* ctx, const float* params, float m_ref, float v_ref,
+                              int32_t enable) {
+  if (!ctx) {
+    set_error("mpm_ctx_set_corrector: null context");
+    return MPM_ERR_ARG;
+  }
+  return ctx->eng.set_corrector(params, m_ref, v_ref, enable);
+}
+
+int32_t mpm_corrector_sample(mpm_ctx* ctx, uint16_t* X, uint32_t* nodes, float* vel, int64_t max_rows,
+                             int64_t* rows) {
+  if (!ctx || (X && (!nodes || !vel || max_rows < 0))) {
+    set_error("mpm_corrector_sample: bad argument");
+    return MPM_ERR_ARG;
+  }
+  return ctx->eng.corrector_sample(X, nodes, vel, max_rows, rows);
+}
+
+int32_t mpm_grid_velocity(mpm_ctx* ctx, const uint32_t* nodes, int64_t count, float* out) {
+  if (!ctx || count < 0 || (count > 0 && (!nodes || !out))) {
+    set_error("mpm_grid_velocity: bad argument");
+    return MPM_ERR_ARG;
+  }
+  return ctx->eng.grid_velocity(nodes, count, out);
+}
+
 int32_t mpm_mlp_sizes(int64_t* n_params, int64_t* n_packed) {
   if (!n_params || !n_packed) {
     set_error("mpm_mlp_sizes: null output");

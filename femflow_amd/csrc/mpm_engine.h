@@ -23,6 +23,13 @@ struct GraphKey {
   }
 };
 
+struct CorrExtract {
+  uint16_t* X;      // rows x 352 BF16 features
+  uint32_t* nodes;  // global node index (block * 64 + node in block)
+  float* vel;       // rows x 3 updated grid velocity
+  int64_t max_rows;
+};
+
 struct TimedLaunch {
   int kind;  // 0 P2G, 1 G2P2G, 2 G2P, 3 grid update
   hipEvent_t e0, e1;
@@ -61,6 +68,14 @@ struct Engine {
   uint32_t* d_sorted_keys = nullptr;
   uint32_t* d_hard = nullptr;
   float4* d_stage = nullptr;
+  // learned corrector
+  bool corr_on = false;
+  float corr_mref = 1.f, corr_vref = 1.f;
+  float4* d_pm = nullptr;
+  float4* d_cellf = nullptr;
+  float* d_corr_params = nullptr;
+  uint16_t* d_corr_packed = nullptr;
+  uint32_t* d_corr_count = nullptr;
   uint32_t stage_cap = 0;
   uint32_t *d_bfirst = nullptr, *d_bcount = nullptr;
   void* d_ws = nullptr;
@@ -80,6 +95,14 @@ struct Engine {
   int positions(float* x);
   void launch_transfer(int mode, hipStream_t s, const EngineArgs& a);
   void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k);
+  void begin_segment(hipStream_t s, int len, int with_sums);
+  void fill_u32(hipStream_t s, uint32_t* p, int64_t count, uint32_t v);
+  void launch_cellf(hipStream_t s, const EngineArgs& a);
+  // corrector after a grid update; ex != null: write features instead (sampling)
+  void launch_corrector(hipStream_t s, const EngineArgs& a, const struct CorrExtract* ex);
+  int set_corrector(const float* params, float m_ref, float v_ref, int enable);
+  int corrector_sample(uint16_t* X, uint32_t* nodes, float* vel, int64_t max_rows, int64_t* rows);
+  int grid_velocity(const uint32_t* nodes, int64_t count, float* out);
   hipEvent_t tevent();
   int run(int nsteps, double* sums, float* frames = nullptr);
   int take_counter(int which, int64_t* out);

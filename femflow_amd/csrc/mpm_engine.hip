@@ -728,6 +728,7 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
         q.z += pv.z;
         q.w += pv.w;
       }
+      if (a.pm) a.pm[addr] = q;
       const float m = q.w;
       float v[3] = {q.x, q.y, q.z};
       if (with_sums) {
@@ -1147,6 +1148,8 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.stage_cap = stage_cap;
   a.bfirst = d_bfirst;
   a.bcount = d_bcount;
+  a.pm = corr_on ? d_pm : nullptr;
+  a.cellf = corr_on ? d_cellf : nullptr;
   return a;
 }
 
@@ -1271,6 +1274,14 @@ void Engine::enqueue_frame(hipStream_t s, float* frames, int64_t step) {
                      frames + (size_t)step * 3 * (size_t)n);
 }
 
+void Engine::begin_segment(hipStream_t s, int len, int with_sums) {
+  hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, with_sums);
+}
+
+void Engine::fill_u32(hipStream_t s, uint32_t* p, int64_t count, uint32_t v) {
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(count, 256)), dim3(256), 0, s, p, count, v);
+}
+
 void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps, float* frames,
                              int64_t step0) {
   const int ws = sums ? 1 : 0;
@@ -1278,11 +1289,15 @@ void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps
   enqueue_bin(s);
   // substep k appends to active list k % 2; its grid update clears the other
   launch_transfer(0, s, args(sums, max_steps, 0));
+  if (corr_on) launch_cellf(s, args(sums, max_steps, 0));
   launch_grid(s, args(sums, max_steps, 0), ws, len, 0);
+  if (corr_on) launch_corrector(s, args(sums, max_steps, 0), nullptr);
   for (int k = 1; k < len; ++k) {
     launch_transfer(1, s, args(sums, max_steps, k & 1));
     enqueue_frame(s, frames, step0 + k - 1);  // substep k-1 finished its G2P
+    if (corr_on) launch_cellf(s, args(sums, max_steps, k & 1));
     launch_grid(s, args(sums, max_steps, k & 1), ws, len, k);
+    if (corr_on) launch_corrector(s, args(sums, max_steps, k & 1), nullptr);
   }
   launch_transfer(2, s, args(sums, max_steps, len & 1));
   enqueue_frame(s, frames, step0 + len - 1);

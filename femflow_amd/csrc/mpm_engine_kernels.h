@@ -82,6 +82,13 @@ struct EngineArgs {
   uint32_t stage_cap;
   const uint32_t* bfirst;
   const uint32_t* bcount;
+  // Learned-corrector features (null when the corrector is off): the grid
+  // update writes each active node's P2G (momentum, mass) to pm; cellf holds,
+  // per node, the sums over the particles whose stencil base is that node of
+  // (m, m F[0..8]) as three float4 (m, mF00, mF01, mF02), (mF10, mF11,
+  // mF12, mF20), (mF21, mF22, 0, 0).
+  float4* pm;
+  float4* cellf;
 };
 
 __device__ __forceinline__ size_t node_addr(const SceneDev& sc, int i, int j, int k) {

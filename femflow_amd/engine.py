@@ -179,6 +179,46 @@ class Engine:
         _lib.check(self._L.mpm_particle_sums(self._h, out), "particle_sums")
         return list(out)
 
+    # -- learned corrector --------------------------------------------------
+    def set_corrector(self, params: Optional[torch.Tensor], m_ref: float = 1.0, v_ref: float = 1.0,
+                      enable: bool = True):
+        """Turn the learned grid corrector on (params: FP32 CUDA MLP parameters,
+        femflow_amd.corrector layout) or off (enable=False)."""
+        self._sync_stream()
+        ptr = None
+        if params is not None:
+            self._check_tensor(params, "params")
+            ptr = params.data_ptr()
+        _lib.check(self._L.mpm_ctx_set_corrector(self._h, ptr, float(m_ref), float(v_ref), 1 if enable else 0),
+                   "set_corrector")
+
+    def corrector_sample(self, max_rows: int):
+        """Features of every node with mass after one P2G + grid update from the
+        current state (particles unchanged): (X (rows, 352) bf16, node ids
+        (rows,) int32 view of uint32, updated velocity (rows, 3))."""
+        self._sync_stream()
+        X = torch.empty((max_rows, 352), dtype=torch.bfloat16, device=self.device)
+        nodes = torch.empty(max_rows, dtype=torch.int32, device=self.device)
+        vel = torch.empty((max_rows, 3), dtype=torch.float32, device=self.device)
+        rows = ct.c_int64()
+        _lib.check(self._L.mpm_corrector_sample(self._h, X.data_ptr(), nodes.data_ptr(), vel.data_ptr(),
+                                                int(max_rows), ct.byref(rows)), "corrector_sample")
+        r = min(rows.value, max_rows)
+        return X[:r], nodes[:r], vel[:r], rows.value
+
+    def p2g_grid_only(self):
+        """One P2G + grid update from the current state; particles unchanged."""
+        self._sync_stream()
+        rows = ct.c_int64()
+        _lib.check(self._L.mpm_corrector_sample(self._h, None, None, None, 0, ct.byref(rows)), "p2g_grid_only")
+
+    def grid_velocity(self, nodes: torch.Tensor) -> torch.Tensor:
+        self._check_tensor(nodes, "nodes", torch.int32)
+        out = torch.empty((nodes.numel(), 3), dtype=torch.float32, device=self.device)
+        _lib.check(self._L.mpm_grid_velocity(self._h, nodes.data_ptr(), nodes.numel(), out.data_ptr()),
+                   "grid_velocity")
+        return out
+
     def kernel_timing(self, enable: bool = True):
         _lib.check(self._L.mpm_ctx_kernel_timing(self._h, 1 if enable else 0), "kernel_timing")
 

@@ -209,6 +209,26 @@ int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particle
                          void* work, float* x_bar, float* v_bar, float* F_bar, float* C_bar,
                          float* mu0_bar, float* lambda0_bar, int64_t* oob_count);
 
+/* ---- learned corrector inside an engine context --------------------------
+ * No reference counterpart.  params: device FP32 MLP parameters (layout of
+ * mpm_mlp_sizes / mpm_mlp_pack); m_ref, v_ref: feature scales (node mass and
+ * velocity).  enable = 1: every substep of mpm_run adds the network's
+ * correction to the updated grid velocity (the context's scene dt is the big
+ * step); 0: plain substeps. */
+int32_t mpm_ctx_set_corrector(mpm_ctx* ctx, const float* params, float m_ref, float v_ref,
+                              int32_t enable);
+/* Training data: one P2G + grid update from the current state (particles
+ * unchanged), then for every node with mass (up to max_rows) its features
+ * (X, rows x 352 BF16), global node index and updated velocity (rows x 3).
+ * X == NULL: only the P2G + grid update and, if enabled, the correction
+ * (then mpm_grid_velocity).  Needs a
+ * corrector set once (enable may be 0).  Synchronises; *rows = nodes found. */
+int32_t mpm_corrector_sample(mpm_ctx* ctx, uint16_t* X, uint32_t* nodes, float* vel, int64_t max_rows,
+                             int64_t* rows);
+/* Updated grid velocity (count x 3) at global node indices from the last
+ * grid update. */
+int32_t mpm_grid_velocity(mpm_ctx* ctx, const uint32_t* nodes, int64_t count, float* out);
+
 /* ---- grid-corrector MLP (BF16 MFMA, FP32 accumulate) -------------------
  * 351 node-stencil features -> 128 -> 128 -> 3, ReLU hidden layers.  No
  * reference counterpart (the reference has no learned component).  Device

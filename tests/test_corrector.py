@@ -112,3 +112,149 @@ def test_mlp_adam_matches_oracle_and_training_reduces_loss(gpu):
     last = float(net.loss_grads(Xg, Tg).item())
     print("training loss", first, "->", last, flush=True)
     assert last < 0.1 * first
+
+
+# -------------------------------------------------- engine integration ----
+def _small_cfg():
+    from femflow_amd.corrector_train import CorrectorConfig
+
+    return CorrectorConfig(n=24000, res=48)
+
+
+def _decode_nodes(nodes, res):
+    """Global node index (block * 64 + node in block) -> (i, j, k), one scene."""
+    nb = (res + 1 + 3) // 4
+    blk, inner = nodes // 64, nodes % 64
+    bi, bj, bk = blk // (nb * nb), (blk // nb) % nb, blk % nb
+    return np.stack([bi * 4 + inner // 16, bj * 4 + (inner // 4) % 4, bk * 4 + inner % 4], axis=1)
+
+
+@pytest.mark.gpu
+def test_engine_features_match_oracle_grid(gpu):
+    """Sampled features = the FP64 oracle's P2G at dt_big (node mass and
+    momentum) plus per-cell mass-weighted mean F, rounded to BF16."""
+    import torch
+
+    from femflow_amd import corrector_train as CT
+    from femflow_amd.scenes import scene_3d_cubes
+    from oracle import mpm_oracle as O
+
+    cfg = _small_cfg()
+    st, pr = scene_3d_cubes(n=cfg.n, res=cfg.res, seed=3)
+    s = CT.scene_tensors(cfg, 3, gpu)
+    gen = CT.SampleGenerator(cfg, gpu)
+    CT._upload(gen.small, s)  # 40 plain substeps first, so that F != I
+    gen.small.run(40)
+    s = CT._download(gen.small, s)
+    CT._upload(gen.big, s)
+    X, nodes, vbig, total = gen.big.corrector_sample(max_rows=gen.max_rows)
+    X = X.float().cpu().numpy()
+    ijk = _decode_nodes(nodes.cpu().numpy().astype(np.int64), cfg.res)
+    # oracle P2G at dt_big from the FP32 state the engine saw
+    ref = {k: s[k].double().cpu().numpy() for k in ("x", "v", "F", "C", "Jp", "mass", "mu0", "lam0")}
+    r1 = cfg.res + 1
+    gv = np.zeros((r1, r1, r1, 3))
+    gm = np.zeros((r1, r1, r1, 1))
+    O.p2g(ref["x"], ref["v"], ref["F"].reshape(-1, 3, 3), ref["C"].reshape(-1, 3, 3), ref["Jp"].reshape(-1, 1),
+          ref["mass"], ref["mu0"], ref["lam0"], st["kind"], st["h"], float(cfg.res), 1.0 / cfg.res, cfg.dt_big,
+          s["volume"], gv, gm)
+    base, _ = O.base_and_fx(ref["x"], float(cfg.res))
+    cm = np.zeros((r1, r1, r1))
+    cF = np.zeros((r1, r1, r1, 9))
+    np.add.at(cm, tuple(base.T), ref["mass"])
+    np.add.at(cF, tuple(base.T), ref["mass"][:, None] * ref["F"].reshape(-1, 9))
+    # nodes whose only contributions round to zero in the engine's fixed point
+    # (weights near 0) carry mass in FP64 only
+    n_ref = int((gm[..., 0] > 0).sum())
+    assert 0.99 * n_ref <= X.shape[0] <= n_ref
+    rng = np.random.default_rng(0)
+    pick = rng.choice(X.shape[0], size=min(400, X.shape[0]), replace=False)
+    eye = np.eye(3).reshape(-1)
+    worst = 0.0
+    for r in pick:
+        i, j, k = ijk[r]
+        feat = []
+        for d0 in (-1, 0, 1):
+            for d1 in (-1, 0, 1):
+                for d2 in (-1, 0, 1):
+                    a, b, c = i + d0, j + d1, k + d2
+                    if not (0 <= a < r1 and 0 <= b < r1 and 0 <= c < r1):
+                        feat += [0.0] * 13
+                        continue
+                    m = gm[a, b, c, 0]
+                    f = [m / cfg.m_ref] + list(gv[a, b, c] / (cfg.m_ref * cfg.v_ref)) if m > 0 else [0.0] * 4
+                    f += list(cF[a, b, c] / cm[a, b, c] - eye) if cm[a, b, c] > 0 else [0.0] * 9
+                    feat += f
+        feat = np.array(feat)
+        err = np.abs(X[r, :351] - feat) - (2.0 ** -7 * np.abs(feat) + 5e-3)
+        worst = max(worst, float(err.max()))
+    gen.close()
+    assert worst <= 0.0, worst
+
+
+@pytest.mark.gpu
+def test_engine_corrector_applies_network_output(gpu):
+    """The fused in-engine MLP adds MLP(features) * v_ref to the updated grid
+    velocity (checked against the standalone forward kernel)."""
+    import torch
+
+    from femflow_amd import corrector_train as CT
+    from femflow_amd.corrector import CorrectorMLP
+
+    cfg = _small_cfg()
+    s = CT.scene_tensors(cfg, 4, gpu)
+    gen = CT.SampleGenerator(cfg, gpu)
+    net = CorrectorMLP(device=gpu, seed=1, out_scale=0.02)
+    CT._upload(gen.big, s)
+    X, nodes, vbig, _ = gen.big.corrector_sample(max_rows=gen.max_rows)
+    X, nodes, vbig = X.clone(), nodes.clone(), vbig.clone()
+    gen.big.set_corrector(net.params, cfg.m_ref, cfg.v_ref, enable=True)
+    CT._upload(gen.big, s)
+    gen.big.p2g_grid_only()
+    vcorr = gen.big.grid_velocity(nodes)
+    pred = net.forward(X).clone() * cfg.v_ref
+    ijk = _decode_nodes(nodes.cpu().numpy().astype(np.int64), cfg.res)
+    inner = torch.as_tensor(((ijk >= 1) & (ijk < cfg.res - 1)).all(1), device=gpu)
+    d = (vcorr - vbig)[inner]
+    p = pred[inner]
+    err = float((d - p).norm() / p.norm())
+    print("in-engine correction vs standalone forward: rel err", err, flush=True)
+    gen.close()
+    assert err < 1e-3
+
+
+@pytest.mark.gpu
+def test_trained_corrector_beats_uncorrected_big_step(gpu):
+    """Ablation: from held-out states of plain rollouts, one corrected big
+    step (dt_big = 8 dt) lands closer to 8 plain substeps than the same big
+    step without the network."""
+    import torch
+
+    from femflow_amd import corrector_train as CT
+    from femflow_amd.corrector import CorrectorMLP
+
+    cfg = _small_cfg()
+    gen = CT.SampleGenerator(cfg, gpu)
+    X, T, w = gen.rollouts(12, 4, 400, 4096, seed=1)
+    net = CorrectorMLP(device=gpu, seed=0)
+    hist = CT.train(net, X, T, w, epochs=12, batch=4096, lr=1e-3)
+    errs = {"plain": [], "corr": []}
+    for seed in (901, 902, 903):
+        s = CT.scene_tensors(cfg, seed, gpu)
+        eng = CT.make_engine(cfg, s["volume"], cfg.dt, cfg.gravity, gpu, s["x"].shape[0])
+        CT._upload(eng, s)
+        eng.run(160)
+        s = CT._download(eng, s)
+        eng.run(cfg.factor)
+        ref = eng.positions().clone()
+        eng.close()
+        for tag, nn in (("plain", None), ("corr", net)):
+            out = CT.CorrectedRunner(cfg, nn, s["volume"], gpu).run(s, 1, judge=False)
+            disp = (ref - s["x"]).norm()
+            errs[tag].append(float((out["x"] - ref).norm() / disp))
+    gen.close()
+    e0, e1 = np.mean(errs["plain"]), np.mean(errs["corr"])
+    print(f"one big step, displacement error vs 8 plain substeps: without network {e0:.4e}, "
+          f"with network {e1:.4e} (val loss {hist['val'][-1]:.3e} vs zero-correction "
+          f"{CT.baseline_loss(T, w):.3e})", flush=True)
+    assert e1 < 0.5 * e0
