@@ -10,6 +10,9 @@ of factor = dt_big / dt) its state s(t) gives one training sample:
     particle's mean velocity over the big step, (x(t + dt_big) - x(t)) /
     dt_big, from `factor` plain substeps, with the wall conditions applied;
   * target correction (v* - v_big) / v_ref, weighted by node mass.
+The applied correction is scaled by min(1, 4 m / m_ref): nodes with little
+mass (surfaces, where the few contributing particles make the features
+noisy) keep the plain big step's velocity.
 A corrected big step then replaces `factor` plain substeps: P2G + grid update
 at dt_big, plus the network's correction, then G2P.
 """
@@ -104,11 +107,21 @@ class SampleGenerator:
         self.ones = torch.ones(self.n, device=device)
         self.max_rows = min(2 * self.n, (cfg.res + 1) ** 3, 1 << 19)
 
-    def sample(self, s: Dict[str, torch.Tensor], rows: int, gen: torch.Generator):
+    def sample(self, s: Dict[str, torch.Tensor], rows: int, gen: torch.Generator, noise_v: float = 0.0,
+               noise_F: float = 0.0):
         """Training rows from state s (caller order); advances self.small by
-        `factor` substeps from s.  Returns X, T, w (node mass / m_ref)."""
+        `factor` substeps from s.  Returns X, T, w (row weights).
+        noise_v, noise_F > 0 perturb the particles' v and F (Gaussian, per
+        entry) for the big step only, while the target comes from the clean
+        state: the network then also learns to pull perturbed states back."""
         cfg = self.cfg
-        _upload(self.big, s)
+        if noise_v > 0 or noise_F > 0:
+            g = torch.Generator(device=self.device).manual_seed(int(torch.randint(0, 2 ** 31, (1,), generator=gen)))
+            vn = s["v"] + noise_v * torch.randn(s["v"].shape, generator=g, device=self.device)
+            Fn = s["F"] + noise_F * torch.randn(s["F"].shape, generator=g, device=self.device)
+            _upload(self.big, s, v=vn.contiguous(), F=Fn.contiguous())
+        else:
+            _upload(self.big, s)
         X, nodes, vbig, total = self.big.corrector_sample(max_rows=self.max_rows)
         if total > self.max_rows:
             raise RuntimeError(f"{total} nodes with mass exceed the sampling buffer ({self.max_rows})")
@@ -120,6 +133,8 @@ class SampleGenerator:
         self.tgt.p2g_grid_only()
         vstar = self.tgt.grid_velocity(nodes)
         T = (vstar - vbig) / cfg.v_ref
+        # weight = node mass / m_ref: the engine scales the applied correction
+        # by min(1, 4 m / m_ref), so light (surface) nodes matter less
         w = X[:, CENTER].float()
         if X.shape[0] > rows:
             idx = torch.randperm(X.shape[0], generator=gen, device="cpu")[:rows].to(self.device)
@@ -127,7 +142,7 @@ class SampleGenerator:
         return X.contiguous(), T.contiguous(), w.contiguous()
 
     def rollouts(self, n_rollouts: int, samples_per_rollout: int, steps: int, rows: int, seed: int = 0,
-                 log: Optional[Callable[[str], None]] = None):
+                 log: Optional[Callable[[str], None]] = None, noise_v: float = 0.0, noise_F: float = 0.0):
         """Rows from n_rollouts seeded rollouts of `steps` plain substeps."""
         cfg = self.cfg
         rng = np.random.default_rng(seed)
@@ -147,7 +162,7 @@ class SampleGenerator:
                     self.small.run(int(ts - t))
                     cur = _download(self.small, cur)
                     t = int(ts)
-                X, T, w = self.sample(cur, rows, gen)
+                X, T, w = self.sample(cur, rows, gen, noise_v, noise_F)
                 Xs.append(X)
                 Ts.append(T)
                 ws.append(w)
@@ -300,3 +315,105 @@ class CorrectedRunner:
 
 def rel_position_error(x, ref) -> float:
     return float((x - ref).norm() / ref.norm())
+
+
+def one_step_gain(cfg: CorrectorConfig, net: CorrectorMLP, device, seeds=(901, 902, 903), warm: int = 160) -> Dict:
+    """From plain-rollout states at substep `warm` of held-out scenes: the
+    displacement error of one big step, with and without the network,
+    against `factor` plain substeps."""
+    errs = {"without": [], "with": []}
+    for seed in seeds:
+        s = scene_tensors(cfg, seed, device)
+        eng = make_engine(cfg, s["volume"], cfg.dt, cfg.gravity, device, s["x"].shape[0])
+        _upload(eng, s)
+        eng.run(warm)
+        s = _download(eng, s)
+        eng.run(cfg.factor)
+        ref = eng.positions().clone()
+        eng.close()
+        disp = (ref - s["x"]).norm()
+        for tag, nn in (("without", None), ("with", net)):
+            out = CorrectedRunner(cfg, nn, s["volume"], device).run(s, 1, judge=False)
+            errs[tag].append(float((out["x"] - ref).norm() / disp))
+    return {k: float(np.mean(v)) for k, v in errs.items()}
+
+
+def rollout_report(cfg: CorrectorConfig, net: Optional[CorrectorMLP], device, seed: int = 0,
+                   checkpoints=(1, 2, 4, 8, 16, 32, 64, 125)) -> Dict:
+    """Big-step rollout from the scene's t = 0 against the plain substep: the
+    relative position error |x - x_plain| / |x_plain| at each checkpoint (in
+    big steps), until a stencil leaves the grid."""
+    s = scene_tensors(cfg, seed, device)
+    n = s["x"].shape[0]
+    plain = make_engine(cfg, s["volume"], cfg.dt, cfg.gravity, device, n)
+    _upload(plain, s)
+    ref, t = {}, 0
+    for k in checkpoints:
+        plain.run(k * cfg.factor - t)
+        t = k * cfg.factor
+        ref[k] = plain.positions().clone()
+    plain.close()
+    eng = make_engine(cfg, s["volume"], cfg.dt_big, cfg.gravity, device, n)
+    if net is not None:
+        eng.set_corrector(net.params, cfg.m_ref, cfg.v_ref, enable=True)
+    _upload(eng, s)
+    out, t = {}, 0
+    try:
+        for k in checkpoints:
+            try:
+                eng.run(k - t, check_oob=True)
+            except RuntimeError:
+                out[str(k)] = "left the grid"
+                break
+            t = k
+            x = eng.positions()
+            e = float((x - ref[k]).norm() / ref[k].norm())
+            out[str(k)] = e if math.isfinite(e) else "non-finite"
+    finally:
+        eng.close()
+    return out
+
+
+def main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="train the grid corrector on solver rollouts")
+    ap.add_argument("--rollouts", type=int, default=200)
+    ap.add_argument("--samples", type=int, default=4, help="sampled big steps per rollout")
+    ap.add_argument("--steps", type=int, default=1000, help="plain substeps per rollout")
+    ap.add_argument("--rows", type=int, default=8192, help="nodes kept per sample")
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=32768)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--noise-v", type=float, default=0.02)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--res", type=int, default=128)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--out", default="corrector.npz")
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda", 0)
+    cfg = CorrectorConfig(n=args.n, res=args.res)
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    gen = SampleGenerator(cfg, dev)
+    X, T, w = gen.rollouts(args.rollouts, args.samples, args.steps, args.rows, seed=args.seed, log=log,
+                           noise_v=args.noise_v)
+    gen.close()
+    log(f"{X.shape[0]} rows from {args.rollouts} rollouts in {time.perf_counter() - t0:.1f} s; "
+        f"zero-correction loss {baseline_loss(T, w):.4e}")
+    net = CorrectorMLP(device=dev, seed=args.seed)
+    t1 = time.perf_counter()
+    hist = train(net, X, T, w, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed, log=log)
+    torch.cuda.synchronize()
+    meta = dict(rollouts=args.rollouts, samples=args.samples, steps=args.steps, rows=int(X.shape[0]),
+                epochs=args.epochs, batch=args.batch, lr=args.lr, noise_v=args.noise_v, n=args.n, res=args.res,
+                dt=cfg.dt, factor=cfg.factor, m_ref=cfg.m_ref, v_ref=cfg.v_ref, seed=args.seed,
+                train_loss=hist["train"], val_loss=hist["val"], zero_loss=baseline_loss(T, w),
+                train_seconds=time.perf_counter() - t1)
+    net.save(args.out, meta)
+    log(json.dumps({k: v for k, v in meta.items() if k not in ("train_loss",)}))
+
+
+if __name__ == "__main__":
+    main()

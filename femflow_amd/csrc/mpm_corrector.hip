@@ -6,8 +6,9 @@
 // and corrector_kernel, one workgroup per active block, gathers the
 // 3x3x3-node stencil features of the block's 64 nodes into LDS, runs the
 // 351 -> 128 -> 128 -> 3 MLP on MFMA (BF16 operands, FP32 accumulate) and
-// adds the predicted correction to the nodes' updated velocities (re-clamped,
-// walls re-applied), then clears the block's cell sums for the next substep.
+// adds the predicted correction, scaled down on nodes with little mass, to
+// the nodes' updated velocities (re-clamped, walls re-applied), then clears
+// the block's cell sums for the next substep.
 // In sampling mode the same kernel writes the features (and the updated
 // velocity) of every node with mass instead: the training data.
 //
@@ -84,6 +85,9 @@ __global__ __launch_bounds__(256) void cellf_kernel(EngineArgs a, int64_t n) {
 
 // ----------------------------------------------------------- corrector ----
 constexpr int XS = 360;  // LDS row stride of the feature tile (BF16)
+// The correction applied to a node is scaled by min(1, m / (CORR_GATE m_ref)):
+// nodes with little mass (surfaces) keep the plain big step's velocity.
+constexpr float CORR_GATE = 0.25f;
 constexpr int HS = 136;  // LDS row stride of the hidden tiles (BF16)
 
 __global__ __launch_bounds__(256) void corrector_kernel(EngineArgs a, CorrArgs c) {
@@ -227,10 +231,12 @@ __global__ __launch_bounds__(256) void corrector_kernel(EngineArgs a, CorrArgs c
         for (int r = 0; r < 4; ++r) {
           const int nr = wv * 16 + (lane >> 4) * 4 + r;
           const size_t na = (size_t)g * 64 + nr;
-          if (!(a.pm[na].w > 0.f)) continue;
+          const float mn = a.pm[na].w;
+          if (!(mn > 0.f)) continue;
           const int I[3] = {bi * 4 + (nr >> 4), bj * 4 + ((nr >> 2) & 3), bk * 4 + (nr & 3)};
           float* vp = reinterpret_cast<float*>(a.vel + na) + li;
-          float v = *vp + (acc[r] + bb) * c.vref;
+          const float gate = tmin(1.f, mn * c.inv_mref * (1.f / CORR_GATE));
+          float v = *vp + gate * (acc[r] + bb) * c.vref;
           v = tmin(tmax(v, -sc.v_allowed), sc.v_allowed);
           if (sc.boundary == MPM_BC_REFERENCE) {
             if (I[li] < 1 || I[li] >= sc.res - 1) v = 0.f;
@@ -306,9 +312,11 @@ int Engine::set_corrector(const float* params, float m_ref, float v_ref, int ena
     if ((rc = alloc((void**)&d_corr_params, sizeof(float) * P_COUNT))) return rc;
     if ((rc = alloc((void**)&d_corr_packed, sizeof(uint16_t) * K_COUNT))) return rc;
     if ((rc = alloc((void**)&d_corr_count, sizeof(uint32_t) * 4))) return rc;
-    if (hipMemset(d_cellf, 0, sizeof(float4) * 3 * nodes) != hipSuccess ||
-        hipMemset(d_pm, 0, sizeof(float4) * nodes) != hipSuccess) {
-      set_error("mpm_ctx_set_corrector: hipMemset failed");
+    // zero on the engine's own (non-blocking) stream, before any of its work
+    fill_u32(own_stream, reinterpret_cast<uint32_t*>(d_cellf), (int64_t)nodes * 12, 0u);
+    fill_u32(own_stream, reinterpret_cast<uint32_t*>(d_pm), (int64_t)nodes * 4, 0u);
+    if (hipStreamSynchronize(own_stream) != hipSuccess) {
+      set_error("mpm_ctx_set_corrector: clearing the feature buffers failed");
       return MPM_ERR_HIP;
     }
   }
@@ -316,7 +324,12 @@ int Engine::set_corrector(const float* params, float m_ref, float v_ref, int ena
   if (params) {
     corr_mref = m_ref;
     corr_vref = v_ref;
-    if (hipMemcpy(d_corr_params, params, sizeof(float) * P_COUNT, hipMemcpyDeviceToDevice) != hipSuccess) {
+    // the caller's parameters are ready on its stream: order the copy after
+    // it, on the engine's stream (a device-to-device hipMemcpy would run on
+    // the null stream, unordered with the engine's non-blocking stream)
+    if (hipEventRecord(ev_in, user_stream) != hipSuccess || hipStreamWaitEvent(own_stream, ev_in, 0) != hipSuccess ||
+        hipMemcpyAsync(d_corr_params, params, sizeof(float) * P_COUNT, hipMemcpyDeviceToDevice, own_stream) !=
+            hipSuccess) {
       set_error("mpm_ctx_set_corrector: parameter copy failed");
       return MPM_ERR_HIP;
     }

@@ -1117,6 +1117,8 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   HIPCHK(hipMemset(d_counters, 0, sizeof(uint32_t) * 16));
   HIPCHK(hipMemset(d_oob, 0, sizeof(unsigned long long) * 2));
   HIPCHK(hipMemset(d_nchunks, 0, sizeof(uint32_t) * 4));
+  // the memsets above run on the null stream; the engine's stream does not
+  // synchronise with it, so wait for them here
   HIPCHK(hipDeviceSynchronize());
   return MPM_OK;
 }

@@ -215,12 +215,15 @@ def test_engine_corrector_applies_network_output(gpu):
     pred = net.forward(X).clone() * cfg.v_ref
     ijk = _decode_nodes(nodes.cpu().numpy().astype(np.int64), cfg.res)
     inner = torch.as_tensor(((ijk >= 1) & (ijk < cfg.res - 1)).all(1), device=gpu)
-    d = (vcorr - vbig)[inner]
-    p = pred[inner]
-    err = float((d - p).norm() / p.norm())
+    mrel = X[:, 13 * 13].float()  # node mass / m_ref
+    full = inner & (mrel >= 0.3)    # correction applied in full (mass gate = 1)
+    light = inner & (mrel < 0.02)   # scaled by 4 m / m_ref < 0.08
+    d = (vcorr - vbig)
+    err = float((d[full] - pred[full]).norm() / pred[full].norm())
     print("in-engine correction vs standalone forward: rel err", err, flush=True)
     gen.close()
     assert err < 1e-3
+    assert bool(light.any()) and bool((d[light].abs() <= 0.1 * pred[light].abs() + 1e-7).all())
 
 
 @pytest.mark.gpu
@@ -258,3 +261,21 @@ def test_trained_corrector_beats_uncorrected_big_step(gpu):
           f"with network {e1:.4e} (val loss {hist['val'][-1]:.3e} vs zero-correction "
           f"{CT.baseline_loss(T, w):.3e})", flush=True)
     assert e1 < 0.5 * e0
+
+
+@pytest.mark.gpu
+def test_shipped_corrector_one_step_gain(gpu):
+    """The shipped network (200 solver rollouts of the headline scene): one
+    corrected big step lands at least 2x closer to 8 plain substeps than the
+    uncorrected big step, on held-out scenes."""
+    import os
+
+    from femflow_amd import corrector_train as CT
+    from femflow_amd.corrector import CorrectorMLP
+
+    path = os.path.join(os.path.dirname(__file__), "..", "femflow_amd", "data", "corrector_c5.npz")
+    net = CorrectorMLP.load(path, device=gpu)
+    assert net.meta["rollouts"] == 200
+    gain = CT.one_step_gain(CT.CorrectorConfig(), net, gpu)
+    print("shipped corrector, one big step:", gain, flush=True)
+    assert gain["with"] < 0.5 * gain["without"]
