@@ -49,6 +49,8 @@ def parse():
                     help="substeps run with per-kernel HIP-event timing")
     ap.add_argument("--batched-steps", type=int, default=500,
                     help="substeps of the 64-scene batched workload (0 skips it)")
+    ap.add_argument("--adjoint", type=int, default=1,
+                    help="run the differentiable-rollout leg (0 skips it)")
     ap.add_argument("--corrector", type=int, default=1,
                     help="run the learned-corrector leg (0 skips it)")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -162,6 +164,52 @@ def corrector_bench(dev):
                     "corrector does not yet keep a 1000-step rollout within 1%"}
 
 
+def adjoint_bench(dev):
+    """Gradient of a 500-substep rollout (2^17 particles, 64^3 grid, E per
+    particle) through the hand-written adjoint, checkpoints every 50."""
+    import numpy as np
+    import torch
+
+    from femflow_amd.adjoint import RolloutConfig, lame_from_E, rollout
+
+    n, res, dt, steps = 1 << 17, 64, 1e-4, 500
+    rng = np.random.default_rng(0)
+    x0 = 0.4 + 0.2 * rng.random((n, 3))
+    v0 = np.tile(0.5 * rng.standard_normal(3), (n, 1))
+    v0[:, 1] -= 3.0 * (x0[:, 1] - 0.5)
+    E = 1e3 + 9e3 * rng.random(n)
+    sub = torch.as_tensor(np.nonzero(x0[:, 1] > 0.5)[0], device=dev)
+    target = torch.tensor(x0[x0[:, 1] > 0.5].mean(0) + 0.05 * rng.standard_normal(3), dtype=torch.float32,
+                          device=dev)
+    vol = 0.2 ** 3 / n
+    cfg = RolloutConfig(res=res, dt=dt, volume=vol, nsteps=steps, checkpoint_every=50)
+    f32 = lambda a: torch.tensor(np.asarray(a), dtype=torch.float32, device=dev)  # noqa: E731
+    mass = f32(np.full(n, vol))
+    v0t, Et = f32(v0).requires_grad_(True), f32(E).requires_grad_(True)
+
+    def once():
+        mu0, lam0 = lame_from_E(Et, 0.2)
+        xT, _ = rollout(f32(x0), v0t, mu0, lam0, mass, cfg)
+        L = ((xT[sub].mean(0) - target) ** 2).sum()
+        return torch.autograd.grad(L, (v0t, Et))
+
+    once()  # warm-up
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    gv, gE = once()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    print(f"[bench] adjoint: forward+backward of {steps} substeps in {ms:.1f} ms", file=sys.stderr, flush=True)
+    return {"workload": "2^17 particles in one elastic cube, E ~ U[1e3, 1e4] per particle, 64^3 grid, dt = 1e-4, "
+                        "500 substeps, checkpoints every 50; loss |centroid of the upper half at T - target|^2; "
+                        "gradient w.r.t. v0 and per-particle E",
+            "ms_forward_backward": ms, "value": n * steps / (ms / 1e3),
+            "unit": "particle-substeps/s (forward + backward)", "dtype": "f32",
+            "grad_norms": {"v0": float(gv.norm()), "E": float(gE.norm())}}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -250,6 +298,7 @@ def main():
     eng.close()
     batched = batched_bench(args.batched_steps, dev) if args.batched_steps > 0 else None
     corrector = corrector_bench(dev) if args.corrector else None
+    adjoint = adjoint_bench(dev) if args.adjoint else None
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -286,6 +335,7 @@ def main():
         "grid_mass_drift_fp64": mass_drift,
         "batched": batched,
         "corrector": corrector,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -509,9 +509,12 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
               const float wt = wij * w2[k];
               const int off = ai + j * ACC_S1 + k;
 #pragma unroll
+              // truncating conversion (one instruction): its bias, under one
+              // unit of 2^-30 of the chunk bound per contribution, is far
+              // below FP32 resolution of the node sums
               for (int d = 0; d < 3; ++d)
-                atomicAdd(&acc_s[d][off], (uint32_t)__float2int_rn(wt * (qij[d] + A2s[d] * (float)k)));
-              atomicAdd(&acc_s[3][off], (uint32_t)__float2int_rn(wt * ms));
+                atomicAdd(&acc_s[d][off], (uint32_t)(int)(wt * (qij[d] + A2s[d] * (float)k)));
+              atomicAdd(&acc_s[3][off], (uint32_t)(int)(wt * ms));
             }
           }
         }
@@ -670,19 +673,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, int with_sums, int len,
                                                                   int k) {
   __shared__ double red[ENG_THREADS / 64][8];
-  __shared__ float4 part[ENG_THREADS / 64 - 1][64];
   const uint32_t count = *a.list_count;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *a.other_count = 0u;  // the list the next substep appends to
     a.counters[3] = 0u;   // deferred list (its pass has run)
   }
   const int t = threadIdx.x;
-  const int grp = t >> 6, inner = t & 63;
-  constexpr int NG = ENG_THREADS / 64;
+  const int wv = t >> 6, inner = t & 63;
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
-  // One active block per workgroup iteration: node `inner`, and the four
-  // waves split the staged chunk arenas (chunk index mod 4) before an LDS sum.
-  for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
+  // One active block per wave, one node per lane.
+  for (uint32_t b = blockIdx.x * 4u + (uint32_t)wv; b < count; b += gridDim.x * 4u) {
     const uint32_t g = a.list[b];
     const SceneDev sc = a.scenes[a.block_scene[g]];
     const int local = (int)g - sc.block_base;
@@ -691,88 +691,95 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
     const int o0 = inner >> 4, o1 = (inner >> 2) & 3, o2 = inner & 3;
     const int i = bi * 4 + o0, j = bj * 4 + o1, k = bk * 4 + o2;
     const size_t addr = (size_t)g * 64 + inner;
-    float4 q = grp == 0 ? a.acc[addr] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // lanes 0..26: first chunk and chunk count of the 27 blocks around this one
+    uint32_t my_c0 = 0, my_cn = 0;
+    if (inner < 27) {
+      const int S0 = bi - 1 + inner / 9, S1 = bj - 1 + (inner / 3) % 3, S2 = bk - 1 + inner % 3;
+      if (S0 >= 0 && S0 < nb && S1 >= 0 && S1 < nb && S2 >= 0 && S2 < nb) {
+        const uint32_t S = (uint32_t)(sc.block_base + (S0 * nb + S1) * nb + S2);
+        my_cn = a.bcount[S];
+        my_c0 = my_cn ? a.bfirst[S] : 0u;
+        my_cn = my_c0 + my_cn <= a.stage_cap ? my_cn : (my_c0 < a.stage_cap ? a.stage_cap - my_c0 : 0u);
+      }
+    }
+    float4 q = a.acc[addr];
     // Staged chunk arenas that cover this node: per axis, node offset o < 3
     // lies in the arenas of blocks b-1 and b, o = 3 in those of b and b+1
-    // (an arena spans nodes 4b-1 .. 4b+6).
-#pragma unroll 1
+    // (an arena spans nodes 4b-1 .. 4b+6).  The first chunk of each of the 8
+    // sources is loaded together; further chunks (dense blocks) follow.
+    uint32_t c0[8], cn[8];
+    int ai[8];
+#pragma unroll
     for (int sn = 0; sn < 8; ++sn) {
       const int s0 = (sn >> 2) - (o0 == 3 ? 0 : 1);
       const int s1 = ((sn >> 1) & 1) - (o1 == 3 ? 0 : 1);
       const int s2 = (sn & 1) - (o2 == 3 ? 0 : 1);
-      const int S0 = bi + s0, S1 = bj + s1, S2 = bk + s2;
-      if (S0 < 0 || S0 >= nb || S1 < 0 || S1 >= nb || S2 < 0 || S2 >= nb) continue;
-      const uint32_t S = (uint32_t)(sc.block_base + (S0 * nb + S1) * nb + S2);
-      const uint32_t cn = a.bcount[S];
-      if (cn == 0u) continue;
-      const uint32_t c0 = a.bfirst[S];
-      const uint32_t c1 = c0 + cn < a.stage_cap ? c0 + cn : a.stage_cap;
-      const float4* src = a.stage + ((o0 + 1 - 4 * s0) * 8 + (o1 + 1 - 4 * s1)) * 8 + (o2 + 1 - 4 * s2);
-#pragma unroll 4
-      for (uint32_t c = c0 + (uint32_t)grp; c < c1; c += NG) {
-        const float4 sv = src[(size_t)c * ARENA_NODES];
-        q.x += sv.x;
-        q.y += sv.y;
-        q.z += sv.z;
-        q.w += sv.w;
+      const int ti = (s0 + 1) * 9 + (s1 + 1) * 3 + (s2 + 1);
+      c0[sn] = (uint32_t)__shfl((int)my_c0, ti);
+      cn[sn] = (uint32_t)__shfl((int)my_cn, ti);
+      ai[sn] = ((o0 + 1 - 4 * s0) * 8 + (o1 + 1 - 4 * s1)) * 8 + (o2 + 1 - 4 * s2);
+    }
+    float4 sv[8];
+#pragma unroll
+    for (int sn = 0; sn < 8; ++sn)
+      sv[sn] = cn[sn] ? a.stage[(size_t)c0[sn] * ARENA_NODES + ai[sn]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int sn = 0; sn < 8; ++sn) {
+      q.x += sv[sn].x;
+      q.y += sv[sn].y;
+      q.z += sv[sn].z;
+      q.w += sv[sn].w;
+      for (uint32_t c = 1; c < cn[sn]; ++c) {
+        const float4 v2 = a.stage[(size_t)(c0[sn] + c) * ARENA_NODES + ai[sn]];
+        q.x += v2.x;
+        q.y += v2.y;
+        q.z += v2.z;
+        q.w += v2.w;
       }
     }
-    if (grp > 0) part[grp - 1][inner] = q;
-    __syncthreads();
-    if (grp == 0) {
-#pragma unroll
-      for (int gg = 0; gg < NG - 1; ++gg) {
-        const float4 pv = part[gg][inner];
-        q.x += pv.x;
-        q.y += pv.y;
-        q.z += pv.z;
-        q.w += pv.w;
-      }
-      if (a.pm) a.pm[addr] = q;
-      const float m = q.w;
-      float v[3] = {q.x, q.y, q.z};
-      if (with_sums) {
-        s[0] += m;
-        s[1] += q.x;
-        s[2] += q.y;
-        s[3] += q.z;
-      }
-      if (m > 0.f) {
-        const float im = 1.f / m;
-        v[0] *= im;
-        v[1] *= im;
-        v[2] *= im;
-        v[1] += sc.dt * sc.gravity;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) v[d] = tmin(tmax(v[d], -sc.v_allowed), sc.v_allowed);
-      }
-      const int I[3] = {i, j, k};
-      if (sc.boundary == MPM_BC_REFERENCE) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          if (I[d] < 1 || I[d] >= sc.res - 1) v[d] = 0.f;
-      } else {
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-          if (I[d] < sc.bound || I[d] > sc.res - sc.bound) v[0] = v[1] = v[2] = 0.f;
-      }
-      if (i > sc.res || j > sc.res || k > sc.res) v[0] = v[1] = v[2] = 0.f;
-      a.vel[addr] = make_float4(v[0], v[1], v[2], m);
-      a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (with_sums) {
-        s[4] += (double)m * v[0];
-        s[5] += (double)m * v[1];
-        s[6] += (double)m * v[2];
-      }
-      if (inner == 0) a.flags[g] = 0u;
+    if (a.pm) a.pm[addr] = q;
+    const float m = q.w;
+    float v[3] = {q.x, q.y, q.z};
+    if (with_sums) {
+      s[0] += m;
+      s[1] += q.x;
+      s[2] += q.y;
+      s[3] += q.z;
     }
-    __syncthreads();
+    if (m > 0.f) {
+      const float im = 1.f / m;
+      v[0] *= im;
+      v[1] *= im;
+      v[2] *= im;
+      v[1] += sc.dt * sc.gravity;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) v[d] = tmin(tmax(v[d], -sc.v_allowed), sc.v_allowed);
+    }
+    const int I[3] = {i, j, k};
+    if (sc.boundary == MPM_BC_REFERENCE) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (I[d] < 1 || I[d] >= sc.res - 1) v[d] = 0.f;
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        if (I[d] < sc.bound || I[d] > sc.res - sc.bound) v[0] = v[1] = v[2] = 0.f;
+    }
+    if (i > sc.res || j > sc.res || k > sc.res) v[0] = v[1] = v[2] = 0.f;
+    a.vel[addr] = make_float4(v[0], v[1], v[2], m);
+    a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (with_sums) {
+      s[4] += (double)m * v[0];
+      s[5] += (double)m * v[1];
+      s[6] += (double)m * v[2];
+    }
+    if (inner == 0) a.flags[g] = 0u;
   }
   if (with_sums) {
 #pragma unroll
     for (int c = 0; c < 7; ++c) {
       const double r = wave_sum(s[c]);
-      if ((t & 63) == 0) red[grp][c] = r;
+      if ((t & 63) == 0) red[wv][c] = r;
     }
   }
   __syncthreads();
@@ -1242,11 +1249,14 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
     hipEventRecord(e1, s);
     tpairs.push_back({mode, e0, e1});
   }
-  hipLaunchKernelGGL(deferred_kernel, dim3(num_cus), dim3(ENG_THREADS), 0, s, a);
+  // the deferred list is short (near-degenerate particles): a small grid keeps
+  // the usually empty pass cheap
+  hipLaunchKernelGGL(deferred_kernel, dim3(32), dim3(ENG_THREADS), 0, s, a);
 }
 
 void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k) {
-  const int g = num_cus * 8;
+  // with sums every workgroup ends with 7 FP64 atomics on the same 7 words
+  const int g = with_sums ? num_cus : num_cus * 8;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (timing) {
     e0 = tevent();
