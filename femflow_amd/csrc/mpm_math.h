@@ -27,6 +27,16 @@ template <typename T> MPM_DEV T tsqrt(T x);
 template <> MPM_DEV float tsqrt<float>(float x) { return __builtin_sqrtf(x); }
 template <> MPM_DEV double tsqrt<double>(double x) { return __builtin_sqrt(x); }
 template <typename T> MPM_DEV T tabs(T x) { return x < T(0) ? -x : x; }
+// Reciprocal and reciprocal square root: FP32 uses the hardware
+// approximations (v_rcp_f32, v_rsq_f32; ~1 ulp), enough for the rotation
+// and scaling factors below, whose exactness does not move the converged
+// results; FP64 stays exact.
+template <typename T> MPM_DEV T trcp(T x);
+template <> MPM_DEV float trcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> MPM_DEV double trcp<double>(double x) { return 1.0 / x; }
+template <typename T> MPM_DEV T trsq(T x);
+template <> MPM_DEV float trsq<float>(float x) { return __builtin_amdgcn_rsqf(x); }
+template <> MPM_DEV double trsq<double>(double x) { return 1.0 / __builtin_sqrt(x); }
 template <typename T> MPM_DEV T tmin(T a, T b) { return a < b ? a : b; }
 template <typename T> MPM_DEV T tmax(T a, T b) { return a > b ? a : b; }
 template <typename T> MPM_DEV T texp(T x);
@@ -97,9 +107,9 @@ template <typename T> MPM_DEV void jacobi_eig3(T A[9], T V[9], int sweeps) {
       const int q = pq == 0 ? 1 : 2;
       const T apq = A[3 * p + q];
       if (apq == T(0)) continue;
-      const T theta = (A[3 * q + q] - A[3 * p + p]) / (T(2) * apq);
-      const T t = (theta >= T(0) ? T(1) : T(-1)) / (tabs(theta) + tsqrt(theta * theta + T(1)));
-      const T c = T(1) / tsqrt(t * t + T(1));
+      const T theta = (A[3 * q + q] - A[3 * p + p]) * trcp(T(2) * apq);
+      const T t = (theta >= T(0) ? T(1) : T(-1)) * trcp(tabs(theta) + tsqrt(theta * theta + T(1)));
+      const T c = trsq(t * t + T(1));
       const T sn = t * c;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
@@ -119,6 +129,10 @@ template <typename T> MPM_DEV void jacobi_eig3(T A[9], T V[9], int sweeps) {
         V[3 * k + p] = c * vkp - sn * vkq;
         V[3 * k + q] = sn * vkp + c * vkq;
       }
+    }
+    if constexpr (sizeof(T) == 4) {  // FP32: stop once the off-diagonal is at rounding level
+      const T off = tabs(A[1]) + tabs(A[2]) + tabs(A[5]);
+      if (off <= T(1e-8) * (tabs(A[0]) + tabs(A[4]) + tabs(A[8]))) break;
     }
   }
 }
@@ -195,8 +209,15 @@ template <typename T> MPM_DEV bool polar_newton(const T* F, T R[9], int iters, T
     T c[9];
     cofactor3(X, c);
     const T det = X[0] * c[0] + X[1] * c[1] + X[2] * c[2];
-    const T g = trcbrt(tabs(det));
-    const T a = T(0.5) * g, b = T(0.5) / (g * det);
+    T g;
+    if constexpr (sizeof(T) == 4) {
+      // any positive scale converges to the same factor; exp2(-log2|det|/3)
+      // on the hardware transcendental units is ample
+      g = __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(tabs(det)) * (-1.f / 3.f));
+    } else {
+      g = trcbrt(tabs(det));
+    }
+    const T a = T(0.5) * g, b = T(0.5) * trcp(g * det);
     T delta = T(0);
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
