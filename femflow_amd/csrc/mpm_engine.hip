@@ -22,7 +22,10 @@ __device__ __forceinline__ float weight_at(float fx, int i) {
 }
 
 // v = sum w g; C = 4 inv_dx sum w g (ijk - fx)^T  (three_d/g2p.py:29-46),
-// accumulated as C[:, a] = 4 inv_dx (sum w g ijk_a - fx_a v).
+// accumulated as C[:, a] = 4 inv_dx (sum w g ijk_a - fx_a v).  The sums are
+// taken axis by axis (w = wi wj wk): per row (i, j) the k-sums
+// sum wk g and sum k wk g, per slab i their j-weighted sums, so each node
+// costs 5 FMAs instead of 13.
 template <bool FROM_LDS>
 __device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int abase,
                                        const float4* __restrict__ vel, const SceneDev& sc,
@@ -47,33 +50,46 @@ __device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int aba
     }
   }
   const size_t gbase = (size_t)sc.block_base * 64;
-  // the outer axis stays a rolled loop: 9 node loads in flight instead of 27
-  // keeps the register footprint (and so the occupancy) of the callers sane
+  const float w2k2 = 2.f * w[2][2];
 #pragma unroll 1
   for (int i = 0; i < 3; ++i) {
     const float wi = weight_at(fx[0], i);
-    const float fi = (float)i;
     size_t Xi = 0;
     if (!FROM_LDS) Xi = i == 0 ? X[0] : (i == 1 ? X[1] : X[2]);
     const int ai = abase + i * 64;
+    float Vi[3] = {0.f, 0.f, 0.f}, Tj[3] = {0.f, 0.f, 0.f}, Tk[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const float wij = wi * w[1][j];
+      float s[3], tk[3];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         float4 g;
         if (FROM_LDS) g = vel_s[ai + j * 8 + k];
         else g = vel[gbase + Xi + Y[j] + Z[k]];
-        const float wt = wij * w[2][k];
-        const float wg[3] = {wt * g.x, wt * g.y, wt * g.z};
+        const float gg[3] = {g.x, g.y, g.z};
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          v[d] += wg[d];
-          S[0][d] += fi * wg[d];
-          if (j) S[1][d] += (float)j * wg[d];
-          if (k) S[2][d] += (float)k * wg[d];
+          if (k == 0) s[d] = w[2][0] * gg[d];
+          else s[d] += w[2][k] * gg[d];
+          if (k == 1) tk[d] = w[2][1] * gg[d];
+          if (k == 2) tk[d] += w2k2 * gg[d];
         }
       }
+      const float wj = w[1][j], jwj = (float)j * w[1][j];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        Vi[d] += wj * s[d];
+        if (j) Tj[d] += jwj * s[d];
+        Tk[d] += wj * tk[d];
+      }
+    }
+    const float iwi = (float)i * wi;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      v[d] += wi * Vi[d];
+      S[0][d] += iwi * Vi[d];
+      S[1][d] += wi * Tj[d];
+      S[2][d] += wi * Tk[d];
     }
   }
   const float c4 = 4.f * sc.inv_dx;
@@ -263,12 +279,14 @@ __device__ __forceinline__ void storek(float* f, int64_t p, const float o[K]) {
 // thread first computes its particles' P2G terms; the chunk sums, per
 // channel, every particle's bound on its largest single-node contribution
 // (which bounds any node's total from this chunk), scales so that sum maps to
-// 2^30, rounds each contribution to int32 and adds it with 32-bit integer LDS
-// atomics (float and 64-bit LDS atomics are an order of magnitude slower on
-// this hardware).  The flush converts each node back to FP32 for the global
-// atomics.  Integer sums are associative, so the in-chunk accumulation is
-// order independent.  Particles that left the arena (drift since binning)
-// take the global-atomic path.
+// 2^30, rounds each contribution to int32 and adds two channels at a time
+// with 64-bit integer LDS atomics (float LDS atomics are an order of
+// magnitude slower on this hardware; one 64-bit add costs less than two
+// 32-bit ones).  Integer sums are associative, so the in-chunk accumulation
+// is order independent.  The flush converts each node back to FP32 and
+// stores the arena's 7x7x7 core to the chunk's staging slot, which the grid
+// update sums; the eighth node layer, chunks beyond the staging capacity and
+// particles that left the arena (drift since binning) take global atomics.
 // Persistent: each XCD (workgroup id mod 8) walks a contiguous eighth of the
 // chunk list, so neighbouring chunks share that XCD's L2.
 struct ChunkScale {
@@ -295,11 +313,23 @@ __device__ __forceinline__ void contribution_bound(const float w[3][3], const P2
   b[3] = wm * fabsf(t.m);
 }
 
-// Accumulation arena: node (a0, a1, a2) of the 8x8x8 arena lives at
-// a0*69 + a1*9 + a2, which spreads the 64 cells of a block over the 64 LDS
-// banks with at most two per bank for every stencil offset (a plain 64/8/1
-// layout puts them on 16 banks).
+// Accumulation arena: two 64-bit words per node (x|y and z|m channel pairs,
+// see pack_fixed2); node (a0, a1, a2) of the 8x8x8 arena lives at
+// a0*69 + a1*9 + a2, which spreads the 64 cells of a block over the LDS
+// banks (a plain 64/8/1 layout puts them on a quarter of the banks).
 constexpr int ACC_S0 = 69, ACC_S1 = 9, ACC_NODES = 7 * ACC_S0 + 7 * ACC_S1 + 7 + 1;
+
+// Two int32 fixed-point values in one 64-bit word whose 64-bit sum carries
+// both sums exactly: word = hi * 2^32 + lo (lo sign-extended).  Valid while
+// each sum stays inside int32, which the chunk scale guarantees.
+__device__ __forceinline__ unsigned long long pack_fixed2(int hi, int lo) {
+  const uint32_t h = (uint32_t)(hi + (lo >> 31));
+  return ((unsigned long long)h << 32) | (unsigned long long)(uint32_t)lo;
+}
+__device__ __forceinline__ void unpack_fixed2(unsigned long long w, int& hi, int& lo) {
+  lo = (int)(uint32_t)w;
+  hi = (int)(uint32_t)(w >> 32) - (lo >> 31);
+}
 
 template <int MODE, int P>
 __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
@@ -310,8 +340,8 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   constexpr int ACC_WORDS = MODE == 2 ? 4 : 4 * ACC_NODES;
   __shared__ __align__(16) uint32_t arena_s[VEL_WORDS > ACC_WORDS ? VEL_WORDS : ACC_WORDS];
   float4* const vel_s = reinterpret_cast<float4*>(arena_s);
-  uint32_t(*const acc_s)[MODE == 2 ? 1 : ACC_NODES] =
-      reinterpret_cast<uint32_t(*)[MODE == 2 ? 1 : ACC_NODES]>(arena_s);
+  unsigned long long(*const acc2)[MODE == 2 ? 1 : ACC_NODES] =
+      reinterpret_cast<unsigned long long(*)[MODE == 2 ? 1 : ACC_NODES]>(arena_s);
   __shared__ float red_s[ENG_THREADS / 64][4];
   __shared__ ChunkScale cs;
   __shared__ uint32_t sub_active[27];
@@ -444,12 +474,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         if (lane == 0) red_s[wv][d] = m;
       }
       __syncthreads();
-      for (int e = t; e < ACC_NODES; e += ENG_THREADS) {
-        acc_s[0][e] = 0u;
-        acc_s[1][e] = 0u;
-        acc_s[2][e] = 0u;
-        acc_s[3][e] = 0u;
-      }
+      for (int e = t; e < 2 * ACC_NODES; e += ENG_THREADS) acc2[0][e] = 0ull;
       if (t < 4) {
         float m = 0.f;
 #pragma unroll
@@ -508,13 +533,15 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
             for (int k = 0; k < 3; ++k) {
               const float wt = wij * w2[k];
               const int off = ai + j * ACC_S1 + k;
-#pragma unroll
               // truncating conversion (one instruction): its bias, under one
               // unit of 2^-30 of the chunk bound per contribution, is far
               // below FP32 resolution of the node sums
-              for (int d = 0; d < 3; ++d)
-                atomicAdd(&acc_s[d][off], (uint32_t)(int)(wt * (qij[d] + A2s[d] * (float)k)));
-              atomicAdd(&acc_s[3][off], (uint32_t)(int)(wt * ms));
+              const int c0 = (int)(wt * (qij[0] + A2s[0] * (float)k));
+              const int c1 = (int)(wt * (qij[1] + A2s[1] * (float)k));
+              const int c2 = (int)(wt * (qij[2] + A2s[2] * (float)k));
+              const int c3 = (int)(wt * ms);
+              atomicAdd(&acc2[0][off], pack_fixed2(c1, c0));
+              atomicAdd(&acc2[1][off], pack_fixed2(c3, c2));
             }
           }
         }
@@ -522,19 +549,22 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
       __syncthreads();
       const double inv[4] = {cs.inv[0], cs.inv[1], cs.inv[2], cs.inv[3]};
       const bool staged = c < a.stage_cap;
-      float4* stage = a.stage + (size_t)c * ARENA_NODES;
+      float4* stage = a.stage + (size_t)c * STAGE_NODES;
 #pragma unroll 1
       for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
-        const int ph = (e >> 6) * ACC_S0 + ((e >> 3) & 7) * ACC_S1 + (e & 7);
-        const int q0 = (int)acc_s[0][ph], q1 = (int)acc_s[1][ph], q2 = (int)acc_s[2][ph],
-                  q3 = (int)acc_s[3][ph];
+        const int e0 = e >> 6, e1 = (e >> 3) & 7, e2 = e & 7;
+        const int ph = e0 * ACC_S0 + e1 * ACC_S1 + e2;
+        int q0, q1, q2, q3;
+        unpack_fixed2(acc2[0][ph], q1, q0);
+        unpack_fixed2(acc2[1][ph], q3, q2);
         const float4 val = make_float4((float)((double)q0 * inv[0]), (float)((double)q1 * inv[1]),
                                        (float)((double)q2 * inv[2]), (float)((double)q3 * inv[3]));
-        if (staged) stage[e] = val;
+        const bool to_stage = staged && e0 < STAGE && e1 < STAGE && e2 < STAGE;
+        if (to_stage) stage[(e0 * STAGE + e1) * STAGE + e2] = val;
         if (q0 != 0 || q1 != 0 || q2 != 0 || q3 != 0) {
-          const int gi = O0 + (e >> 6), gj = O1 + ((e >> 3) & 7), gk = O2 + (e & 7);
+          const int gi = O0 + e0, gj = O1 + e1, gk = O2 + e2;
           if (gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res) {
-            if (!staged) {
+            if (!to_stage) {
               float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, gi, gj, gk));
               atomicAdd(dst + 0, val.x);
               atomicAdd(dst + 1, val.y);
@@ -717,12 +747,16 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       const int ti = (s0 + 1) * 9 + (s1 + 1) * 3 + (s2 + 1);
       c0[sn] = (uint32_t)__shfl((int)my_c0, ti);
       cn[sn] = (uint32_t)__shfl((int)my_cn, ti);
-      ai[sn] = ((o0 + 1 - 4 * s0) * 8 + (o1 + 1 - 4 * s1)) * 8 + (o2 + 1 - 4 * s2);
+      // arena index of this node in the source's arena; index 7 (the
+      // unstaged eighth layer) came through the global accumulator
+      const int x0 = o0 + 1 - 4 * s0, x1 = o1 + 1 - 4 * s1, x2 = o2 + 1 - 4 * s2;
+      ai[sn] = (x0 * STAGE + x1) * STAGE + x2;
+      if (x0 == STAGE || x1 == STAGE || x2 == STAGE) cn[sn] = 0u;
     }
     float4 sv[8];
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn)
-      sv[sn] = cn[sn] ? a.stage[(size_t)c0[sn] * ARENA_NODES + ai[sn]] : make_float4(0.f, 0.f, 0.f, 0.f);
+      sv[sn] = cn[sn] ? a.stage[(size_t)c0[sn] * STAGE_NODES + ai[sn]] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn) {
       q.x += sv[sn].x;
@@ -730,7 +764,7 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       q.z += sv[sn].z;
       q.w += sv[sn].w;
       for (uint32_t c = 1; c < cn[sn]; ++c) {
-        const float4 v2 = a.stage[(size_t)(c0[sn] + c) * ARENA_NODES + ai[sn]];
+        const float4 v2 = a.stage[(size_t)(c0[sn] + c) * STAGE_NODES + ai[sn]];
         q.x += v2.x;
         q.y += v2.y;
         q.z += v2.z;
@@ -1106,7 +1140,7 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   const int64_t cap_chunks = (max_chunks < n + 1 ? max_chunks : n + 1);
   const int64_t want = 2 * ((n + 255) / 256) + 8192;
   stage_cap = (uint32_t)(want < cap_chunks ? want : cap_chunks);
-  if ((rc = alloc((void**)&d_stage, sizeof(float4) * ARENA_NODES * (size_t)stage_cap))) return rc;
+  if ((rc = alloc((void**)&d_stage, sizeof(float4) * STAGE_NODES * (size_t)stage_cap))) return rc;
   if ((rc = alloc((void**)&d_bfirst, sizeof(uint32_t) * blocks))) return rc;
   if ((rc = alloc((void**)&d_bcount, sizeof(uint32_t) * blocks))) return rc;
   HIPCHK(hipMemset(d_bcount, 0, sizeof(uint32_t) * blocks));

@@ -19,6 +19,12 @@ namespace mpm {
 constexpr int ENG_THREADS = 256;
 constexpr int ARENA = 8;  // nodes per axis
 constexpr int ARENA_NODES = ARENA * ARENA * ARENA;
+// Staged part of a chunk's arena: nodes 0..6 per axis.  A particle inside
+// its block reaches arena nodes 0..6 only; the eighth layer is reached by
+// particles that drifted out of the block since binning and goes through
+// global atomics.
+constexpr int STAGE = 7;
+constexpr int STAGE_NODES = STAGE * STAGE * STAGE;
 constexpr int MAX_SCENES = 4096;
 
 struct SceneDev {
