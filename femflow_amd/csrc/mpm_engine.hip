@@ -727,8 +727,10 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       const int S0 = bi - 1 + inner / 9, S1 = bj - 1 + (inner / 3) % 3, S2 = bk - 1 + inner % 3;
       if (S0 >= 0 && S0 < nb && S1 >= 0 && S1 < nb && S2 >= 0 && S2 < nb) {
         const uint32_t S = (uint32_t)(sc.block_base + (S0 * nb + S1) * nb + S2);
+        // both loads in flight together (bfirst is stale where bcount is 0)
         my_cn = a.bcount[S];
-        my_c0 = my_cn ? a.bfirst[S] : 0u;
+        my_c0 = a.bfirst[S];
+        my_c0 = my_cn ? my_c0 : 0u;
         my_cn = my_c0 + my_cn <= a.stage_cap ? my_cn : (my_c0 < a.stage_cap ? a.stage_cap - my_c0 : 0u);
       }
     }
@@ -753,17 +755,27 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       ai[sn] = (x0 * STAGE + x1) * STAGE + x2;
       if (x0 == STAGE || x1 == STAGE || x2 == STAGE) cn[sn] = 0u;
     }
-    float4 sv[8];
-#pragma unroll
-    for (int sn = 0; sn < 8; ++sn)
-      sv[sn] = cn[sn] ? a.stage[(size_t)c0[sn] * STAGE_NODES + ai[sn]] : make_float4(0.f, 0.f, 0.f, 0.f);
+    // up to three chunks per source are loaded in one round trip (a block
+    // of ~20 particles per cell has three 512-particle chunks); the sums
+    // keep source-major, chunk-minor order
+    float4 sv[3][8];
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn) {
-      q.x += sv[sn].x;
-      q.y += sv[sn].y;
-      q.z += sv[sn].z;
-      q.w += sv[sn].w;
-      for (uint32_t c = 1; c < cn[sn]; ++c) {
+      const float4* src = a.stage + (size_t)c0[sn] * STAGE_NODES + ai[sn];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        sv[c][sn] = cn[sn] > (uint32_t)c ? src[(size_t)c * STAGE_NODES] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+#pragma unroll
+    for (int sn = 0; sn < 8; ++sn) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        q.x += sv[c][sn].x;
+        q.y += sv[c][sn].y;
+        q.z += sv[c][sn].z;
+        q.w += sv[c][sn].w;
+      }
+      for (uint32_t c = 3; c < cn[sn]; ++c) {
         const float4 v2 = a.stage[(size_t)(c0[sn] + c) * STAGE_NODES + ai[sn]];
         q.x += v2.x;
         q.y += v2.y;
