@@ -28,6 +28,7 @@ EXPORTS = (
     "mpm_run", "mpm_run_frames", "mpm_state_positions", "mpm_substep", "mpm_ctx_oob_count", "mpm_ctx_fallback_count",
     "mpm_particle_sums", "mpm_bin", "mpm_dense_workspace_bytes", "mpm_dense_substep", "mpm_adjoint_step",
     "mpm_ctx_set_corrector", "mpm_corrector_sample", "mpm_grid_velocity",
+    "mpm_collide_planes", "mpm_ctx_set_planes",
     "mpm_mlp_sizes", "mpm_mlp_pack", "mpm_mlp_forward", "mpm_mlp_loss", "mpm_mlp_backward", "mpm_mlp_adam",
 )
 
@@ -86,6 +87,8 @@ def load():
             "mpm_device_count": ([], i32),
             "mpm_p2g": ([vp, Q, P, i32, dbl, M, i32, vp, vp, vp], i32),
             "mpm_grid_op": ([vp, Q, i32, vp, vp], i32),
+            "mpm_collide_planes": ([vp, i32, i32, dbl, i32, vp, vp, i32, vp], i32),
+            "mpm_ctx_set_planes": ([vp, i32, i32, vp, vp, i32], i32),
             "mpm_g2p": ([vp, Q, P, i32, dbl, dbl, M, i32, vp, vp], i32),
             "mpm_ctx_create": ([ct.POINTER(vp), i64, i32, S, i32, M, i32], i32),
             "mpm_ctx_destroy": ([vp], i32),

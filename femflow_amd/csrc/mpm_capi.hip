@@ -238,6 +238,35 @@ int32_t mpm_grid_op(void* stream, const mpm_params* prm, int32_t dtype, void* gr
   return MPM_OK;
 }
 
+int32_t mpm_collide_planes(void* stream, int32_t dtype, int32_t res, double dx, int32_t nplanes,
+                           const double* points, const double* normals, int32_t quirks, void* grid_v) {
+  if (!grid_v || (dtype != MPM_F32 && dtype != MPM_F64) || res < 1 || nplanes < 0 || nplanes > mpm::MAX_PLANES ||
+      (nplanes > 0 && (!points || !normals)) || (quirks != MPM_QUIRKS_REFERENCE && quirks != MPM_QUIRKS_TEXTBOOK)) {
+    set_error("mpm_collide_planes: null grid, bad dtype, res < 1, more than 16 planes or bad quirks mode");
+    return MPM_ERR_ARG;
+  }
+  if (nplanes == 0) return MPM_OK;
+  mpm::PlaneDev planes[mpm::MAX_PLANES];
+  for (int q = 0; q < nplanes; ++q) {
+    if (!mpm::make_plane(points + 3 * q, normals + 3 * q, dx, quirks, 0, planes[q])) {
+      set_error("mpm_collide_planes: a normal is zero or not finite");
+      return MPM_ERR_ARG;
+    }
+  }
+  mpm::phase_planes((hipStream_t)stream, dtype, res, planes, nplanes, grid_v);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
+int32_t mpm_ctx_set_planes(mpm_ctx* ctx, int32_t scene, int32_t nplanes, const double* points,
+                           const double* normals, int32_t quirks) {
+  if (!ctx) {
+    set_error("mpm_ctx_set_planes: null context");
+    return MPM_ERR_ARG;
+  }
+  return ctx->eng.set_planes(scene, nplanes, points, normals, quirks);
+}
+
 int32_t mpm_g2p(void* stream, const mpm_params* prm, mpm_particles* p, int32_t kind_override,
                 double theta_c, double theta_s, const mpm_material* materials, int32_t nmat,
                 const void* grid_v, int64_t* oob_count) {

@@ -76,6 +76,9 @@ struct Engine {
   float* d_corr_params = nullptr;
   uint16_t* d_corr_packed = nullptr;
   uint32_t* d_corr_count = nullptr;
+  // sticky half-space colliders (all scenes, at most MAX_PLANES)
+  std::vector<PlaneDev> planes_host;
+  PlaneDev* d_planes = nullptr;
   uint32_t stage_cap = 0;
   uint32_t *d_bfirst = nullptr, *d_bcount = nullptr;
   void* d_ws = nullptr;
@@ -103,6 +106,7 @@ struct Engine {
   int set_corrector(const float* params, float m_ref, float v_ref, int enable);
   int corrector_sample(uint16_t* X, uint32_t* nodes, float* vel, int64_t max_rows, int64_t* rows);
   int grid_velocity(const uint32_t* nodes, int64_t count, float* out);
+  int set_planes(int scene, int count, const double* points, const double* normals, int quirks);
   hipEvent_t tevent();
   int run(int nsteps, double* sums, float* frames = nullptr);
   int take_counter(int which, int64_t* out);

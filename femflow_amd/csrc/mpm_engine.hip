@@ -714,7 +714,8 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
   // One active block per wave, one node per lane.
   for (uint32_t b = blockIdx.x * 4u + (uint32_t)wv; b < count; b += gridDim.x * 4u) {
     const uint32_t g = a.list[b];
-    const SceneDev sc = a.scenes[a.block_scene[g]];
+    const int sid = a.block_scene[g];
+    const SceneDev sc = a.scenes[sid];
     const int local = (int)g - sc.block_base;
     const int nb = sc.nb;
     const int bi = local / (nb * nb), bj = (local / nb) % nb, bk = local % nb;
@@ -812,6 +813,8 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
         if (I[d] < sc.bound || I[d] > sc.res - sc.bound) v[0] = v[1] = v[2] = 0.f;
     }
     if (i > sc.res || j > sc.res || k > sc.res) v[0] = v[1] = v[2] = 0.f;
+    for (int pl = 0; pl < a.nplanes; ++pl)
+      if (a.planes[pl].scene == sid && plane_blocks(a.planes[pl], i, j, k)) v[0] = v[1] = v[2] = 0.f;
     a.vel[addr] = make_float4(v[0], v[1], v[2], m);
     a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (with_sums) {
@@ -1205,6 +1208,8 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.bcount = d_bcount;
   a.pm = corr_on ? d_pm : nullptr;
   a.cellf = corr_on ? d_cellf : nullptr;
+  a.planes = d_planes;
+  a.nplanes = (int)planes_host.size();
   return a;
 }
 
@@ -1457,6 +1462,53 @@ int Engine::kernel_times(double* ms, int64_t* counts) {
   }
   tpairs.clear();
   tnext = 0;
+  return MPM_OK;
+}
+
+}  // namespace mpm
+
+namespace mpm {
+
+// Replace the sticky half-space colliders of one scene (count 0 removes them).
+// Captured graphs hold the old table size, so they are dropped.
+int Engine::set_planes(int scene, int count, const double* points, const double* normals, int quirks) {
+  if (scene < 0 || scene >= (int)scenes_host.size() || count < 0 || (count > 0 && (!points || !normals)) ||
+      (quirks != MPM_QUIRKS_REFERENCE && quirks != MPM_QUIRKS_TEXTBOOK)) {
+    set_error("mpm_ctx_set_planes: bad scene, count, arrays or quirks mode");
+    return MPM_ERR_ARG;
+  }
+  std::vector<PlaneDev> next;
+  for (const PlaneDev& p : planes_host)
+    if (p.scene != scene) next.push_back(p);
+  const double dx = 1.0 / scenes_host[scene].res;
+  for (int i = 0; i < count; ++i) {
+    PlaneDev p{};
+    if (!make_plane(points + 3 * i, normals + 3 * i, dx, quirks, scene, p)) {
+      set_error("mpm_ctx_set_planes: a normal is zero or not finite");
+      return MPM_ERR_ARG;
+    }
+    next.push_back(p);
+  }
+  if (next.size() > (size_t)MAX_PLANES) {
+    set_error("mpm_ctx_set_planes: more than 16 planes in the context");
+    return MPM_ERR_ARG;
+  }
+  int rc;
+  if (!d_planes && (rc = alloc((void**)&d_planes, sizeof(PlaneDev) * MAX_PLANES))) return rc;
+  if (hipStreamSynchronize(own_stream) != hipSuccess) {
+    set_error("mpm_ctx_set_planes: synchronising the engine stream failed");
+    return MPM_ERR_HIP;
+  }
+  for (auto& kv : graphs) hipGraphExecDestroy(kv.second);
+  graphs.clear();
+  if (!next.empty() &&
+      (hipMemcpyAsync(d_planes, next.data(), sizeof(PlaneDev) * next.size(), hipMemcpyHostToDevice, own_stream) !=
+           hipSuccess ||
+       hipStreamSynchronize(own_stream) != hipSuccess)) {
+    set_error("mpm_ctx_set_planes: copying the plane table failed");
+    return MPM_ERR_HIP;
+  }
+  planes_host = next;
   return MPM_OK;
 }
 

@@ -95,6 +95,9 @@ struct EngineArgs {
   // mF12, mF20), (mF21, mF22, 0, 0).
   float4* pm;
   float4* cellf;
+  // Sticky half-space colliders (null / 0 when none): see PlaneDev.
+  const PlaneDev* planes;
+  int nplanes;
 };
 
 __device__ __forceinline__ size_t node_addr(const SceneDev& sc, int i, int j, int k) {

@@ -427,4 +427,42 @@ template <typename T> MPM_DEV void svd2(const T* F, T U[4], T s[2], T V[4]) {
   s[1] = s2;
 }
 
+// -------------------------------------------------------------------------
+// Sticky half-space collider of three_d/grid_op.py:50-67
+// (check_collision_points): every node I of the plane's scene with
+// dot(I dx - p, n) < 0 gets zero velocity.  n holds the reference's
+// normal + 1/|normal| (the scalar added to each component) or, in textbook
+// mode, the unit normal.  The test runs in FP64 with one rounding per
+// operation, in the order ((o0 n0 + o1 n1) + o2 n2).
+constexpr int MAX_PLANES = 16;
+struct PlaneDev {
+  double p[3], n[3], dx;
+  int scene;
+};
+
+__device__ __forceinline__ bool plane_blocks(const PlaneDev& pl, int i, int j, int k) {
+  const double o0 = __dsub_rn(__dmul_rn((double)i, pl.dx), pl.p[0]);
+  const double o1 = __dsub_rn(__dmul_rn((double)j, pl.dx), pl.p[1]);
+  const double o2 = __dsub_rn(__dmul_rn((double)k, pl.dx), pl.p[2]);
+  const double d = __dadd_rn(__dadd_rn(__dmul_rn(o0, pl.n[0]), __dmul_rn(o1, pl.n[1])), __dmul_rn(o2, pl.n[2]));
+  return d < 0.0;
+}
+
+// Host: the plane as the test above uses it (normal per quirks mode).
+// Returns false for a zero or non-finite normal.
+inline bool make_plane(const double* point, const double* normal, double dx, int quirks, int scene,
+                       PlaneDev& out) {
+#pragma clang fp contract(off)
+  const double s0 = normal[0] * normal[0], s1 = normal[1] * normal[1], s2 = normal[2] * normal[2];
+  const double denom = __builtin_sqrt((s0 + s1) + s2);
+  if (!(denom > 0.0) || !(denom < __builtin_inf())) return false;
+  for (int d = 0; d < 3; ++d) {
+    out.p[d] = point[d];
+    out.n[d] = quirks == MPM_QUIRKS_TEXTBOOK ? normal[d] / denom : normal[d] + 1.0 / denom;
+  }
+  out.dx = dx;
+  out.scene = scene;
+  return true;
+}
+
 }  // namespace mpm

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "mpm355.h"
+#include "mpm_math.h"
 
 #define MPM_PHASE_MAX_MATERIALS 16
 
@@ -13,6 +14,7 @@ void phase_p2g(hipStream_t st, const mpm_params* prm, const mpm_particles* p, in
                double hardening, const mpm_material* mats, int nmat, void* gv, void* gm,
                unsigned long long* oob);
 void phase_grid_op(hipStream_t st, const mpm_params* prm, int dtype, void* gv, const void* gm);
+void phase_planes(hipStream_t st, int dtype, int res, const PlaneDev* planes, int n, void* gv);
 void phase_g2p(hipStream_t st, const mpm_params* prm, mpm_particles* p, int kind_override,
                double theta_c, double theta_s, const mpm_material* mats, int nmat, const void* gv,
                unsigned long long* oob);

@@ -177,6 +177,28 @@ __global__ void phase_grid_kernel(PhaseArgs<T> a) {
   }
 }
 
+// three_d/grid_op.py:50-67 (check_collision_points), one thread per node of
+// the dense (R+1)^3 grid; every plane is tested in turn.
+struct PlaneSet {
+  PlaneDev p[MAX_PLANES];
+  int n;
+};
+
+template <typename T>
+__global__ void phase_planes_kernel(T* __restrict__ gv, int res, PlaneSet ps) {
+  const int64_t r1 = res + 1;
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= r1 * r1 * r1) return;
+  const int i = (int)(node / (r1 * r1)), j = (int)((node / r1) % r1), k = (int)(node % r1);
+  bool hit = false;
+  for (int q = 0; q < ps.n; ++q) hit = hit || plane_blocks(ps.p[q], i, j, k);
+  if (hit) {
+    gv[3 * node] = T(0);
+    gv[3 * node + 1] = T(0);
+    gv[3 * node + 2] = T(0);
+  }
+}
+
 template <typename T, int DIM>
 __global__ void phase_g2p_kernel(PhaseArgs<T> a) {
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -391,6 +413,16 @@ void phase_grid_op(hipStream_t st, const mpm_params* prm, int dtype, void* gv, c
     if (prm->dim == 3) hipLaunchKernelGGL((phase_grid_kernel<float, 3>), dim3(nb), dim3(256), 0, st, a);
     else hipLaunchKernelGGL((phase_grid_kernel<float, 2>), dim3(nb), dim3(256), 0, st, a);
   }
+}
+
+void phase_planes(hipStream_t st, int dtype, int res, const PlaneDev* planes, int n, void* gv) {
+  PlaneSet ps{};
+  for (int q = 0; q < n; ++q) ps.p[q] = planes[q];
+  ps.n = n;
+  const int64_t r1 = res + 1;
+  const unsigned nb = blocks_for(r1 * r1 * r1, 256);
+  if (dtype == MPM_F64) hipLaunchKernelGGL(phase_planes_kernel<double>, dim3(nb), dim3(256), 0, st, (double*)gv, res, ps);
+  else hipLaunchKernelGGL(phase_planes_kernel<float>, dim3(nb), dim3(256), 0, st, (float*)gv, res, ps);
 }
 
 void phase_g2p(hipStream_t st, const mpm_params* prm, mpm_particles* p, int kind_override,

@@ -14,6 +14,7 @@ import sys
 from dataclasses import dataclass
 from typing import Optional, Sequence
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -178,6 +179,21 @@ class Engine:
         out = (ct.c_double * 7)()
         _lib.check(self._L.mpm_particle_sums(self._h, out), "particle_sums")
         return list(out)
+
+    # -- plane colliders ----------------------------------------------------
+    def set_planes(self, points, normals, scene: int = 0, quirks: str = "reference"):
+        """Sticky half-space colliders for one scene (three_d/grid_op.py:50-67),
+        applied by every grid update after the walls; replaces the scene's
+        previous planes (empty arrays remove them).  points, normals: (K, 3)
+        in the unit-cube coordinates of the grid; at most 16 per engine."""
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+        nrm = np.ascontiguousarray(np.asarray(normals, dtype=np.float64).reshape(-1, 3))
+        if pts.shape != nrm.shape:
+            raise ValueError("points and normals must have the same number of rows")
+        q = _lib.MPM_QUIRKS_TEXTBOOK if quirks == "textbook" else _lib.MPM_QUIRKS_REFERENCE
+        self._sync_stream()
+        _lib.check(self._L.mpm_ctx_set_planes(self._h, int(scene), len(pts), pts.ctypes.data if len(pts) else None,
+                                              nrm.ctypes.data if len(nrm) else None, q), "set_planes")
 
     # -- learned corrector --------------------------------------------------
     def set_corrector(self, params: Optional[torch.Tensor], m_ref: float = 1.0, v_ref: float = 1.0,

@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes as ct
 
+import numpy as np
 import torch
 
 from ... import _lib
@@ -120,6 +121,28 @@ def run_grid_op(prm, grid_velocity, grid_mass):
     gv = Staged(grid_velocity, dt, dev)
     gm = Staged(grid_mass, dt, dev, writable=False)
     _lib.check(L.mpm_grid_op(stream_ptr(dev), ct.byref(prm), mpm_dtype(dt), gv.ptr, gm.ptr), "grid_op")
+    gv.commit()
+
+
+def plane_arrays(points, normals):
+    """(K, 3) float64 host arrays of plane points and normals, K <= 16."""
+    pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64).reshape(-1, 3))
+    nrm = np.ascontiguousarray(np.asarray(normals, dtype=np.float64).reshape(-1, 3))
+    if pts.shape != nrm.shape:
+        raise ValueError("points and normals must have the same number of rows")
+    return pts, nrm
+
+
+def run_collide_planes(points, normals, res, dx, grid_velocity, quirks):
+    _lib.require_gpu()
+    L = _lib.load()
+    pts, nrm = plane_arrays(points, normals)
+    dt = compute_dtype(grid_velocity)
+    dev = device_of(grid_velocity)
+    gv = Staged(grid_velocity, dt, dev)
+    q = _lib.MPM_QUIRKS_TEXTBOOK if quirks == "textbook" else _lib.MPM_QUIRKS_REFERENCE
+    _lib.check(L.mpm_collide_planes(stream_ptr(dev), mpm_dtype(dt), int(res), float(dx), len(pts),
+                                    pts.ctypes.data, nrm.ctypes.data, q, gv.ptr), "check_collision_points")
     gv.commit()
 
 

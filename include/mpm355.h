@@ -118,6 +118,16 @@ int32_t mpm_p2g(void* stream, const mpm_params* prm, const mpm_particles* p,
 int32_t mpm_grid_op(void* stream, const mpm_params* prm, int32_t dtype, void* grid_v,
                     const void* grid_m);
 
+/* Replaces three_d.grid_op.check_collision_points (three_d/grid_op.py:50-67):
+ * sticky half-space colliders on a dense 3D grid_v ((R+1)^3 x 3, dtype
+ * MPM_F32/MPM_F64, device).  points, normals: host (nplanes x 3) doubles,
+ * nplanes <= 16.  Every node I with dot(I dx - point, n) < 0 gets v = 0,
+ * where n = normal + 1/|normal| (MPM_QUIRKS_REFERENCE: the reference adds the
+ * scalar to each component) or normal/|normal| (MPM_QUIRKS_TEXTBOOK).  The
+ * test is evaluated in FP64. */
+int32_t mpm_collide_planes(void* stream, int32_t dtype, int32_t res, double dx, int32_t nplanes,
+                           const double* points, const double* normals, int32_t quirks, void* grid_v);
+
 /* Replaces three_d.g2p (three_d/g2p.py:9-59) and two_d.g2p (two_d/g2p.py:5-47).
  * kind_override >= 0: MPM_SNOW clamps with theta_c/theta_s for every
  * particle (model == "snow"), MPM_ELASTIC never clamps. */
@@ -217,6 +227,14 @@ int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particle
  * step); 0: plain substeps. */
 int32_t mpm_ctx_set_corrector(mpm_ctx* ctx, const float* params, float m_ref, float v_ref,
                               int32_t enable);
+/* Sticky half-space colliders of one scene of the engine, applied by every
+ * grid update after the walls (the rule of mpm_collide_planes; the engine
+ * grid spans [0, 1]^3 with dx = 1/res).  Replaces the scene's previous
+ * planes; nplanes = 0 removes them; at most 16 planes per context.  The
+ * reference defines the collider (three_d/grid_op.py:50-67) but its step
+ * never calls it. */
+int32_t mpm_ctx_set_planes(mpm_ctx* ctx, int32_t scene, int32_t nplanes, const double* points,
+                           const double* normals, int32_t quirks);
 /* Training data: one P2G + grid update from the current state (particles
  * unchanged), then for every node with mass (up to max_rows) its features
  * (X, rows x 352 BF16), global node index and updated velocity (rows x 3).

@@ -50,6 +50,8 @@ PARITY = {
     "g2p_3d": "linear field reproduction (v = a + B x, C = B); C++ restatement",
     "bin_keys": "brute-force integer arithmetic; truncation toward zero",
     "bin_sort": "numpy.argsort(kind='stable') (library routine)",
+    "collide_planes": "closed-form node counts (axis-aligned textbook planes; a hand-counted "
+                      "reference-mode plane); per-node loop restatement in plain Python floats",
 }
 
 
@@ -243,6 +245,31 @@ def grid_op_3d(res, dx, dt, gravity, grid_v, grid_m, boundary="reference", bound
         gv[:, :, wall] = 0
 
 
+def plane_normal(normal, quirks="reference"):
+    """three_d/grid_op.py:59-60: n + 1/|n| (the scalar added to every
+    component; reference) or n/|n| (textbook)."""
+    n = np.asarray(normal, dtype=np.float64)
+    denom = np.sqrt((n[0] * n[0] + n[1] * n[1]) + n[2] * n[2])
+    return n + 1.0 / denom if quirks == "reference" else n / denom
+
+
+def collide_planes(points, normals, res, dx, grid_v, quirks="reference"):
+    """Sticky half-space colliders, three_d/grid_op.py:50-67: for each plane,
+    every node I with dot(I dx - point, n) < 0 gets v = 0 (all components).
+    The dot product is taken as ((o0 n0 + o1 n1) + o2 n2), each operation
+    rounded once (the reference's np.dot may order its three terms
+    differently, which can only matter for nodes on the plane itself)."""
+    idx = np.arange(res + 1, dtype=np.float64) * dx
+    for point, normal in zip(np.asarray(points, np.float64).reshape(-1, 3),
+                             np.asarray(normals, np.float64).reshape(-1, 3)):
+        n = plane_normal(normal, quirks)
+        o0 = (idx - point[0])[:, None, None]
+        o1 = (idx - point[1])[None, :, None]
+        o2 = (idx - point[2])[None, None, :]
+        d = (o0 * n[0] + o1 * n[1]) + o2 * n[2]
+        grid_v[d < 0] = 0.0
+
+
 def grid_op_2d(res, dt, gravity, grid_v, grid_m, boundary="reference", bound=3):
     """2D: only nodes with m>0: v/=m, v_y += dt g; sticky if x<0.05, x>0.95 or
     y>0.95; floor y<0.05 keeps v_y>=0 (two_d/grid_op.py:13-24).
@@ -404,7 +431,9 @@ def substep(state, params, quirks="reference", svd_gauge="consistent", return_gr
 
     state: x, v, F, C, Jp, mass, mu0, lam0, kind (per particle) and per-particle
     h, theta_c, theta_s.  params: res, dx, inv_dx, dt, volume, gravity,
-    boundary ("reference"|"sticky"), bound (sticky thickness in nodes)."""
+    boundary ("reference"|"sticky"), bound (sticky thickness in nodes),
+    optional planes = (points, normals, quirks) applied after the 3D grid
+    update (collide_planes)."""
     x = state["x"]
     n, dim = x.shape
     res = params["res"]
@@ -420,6 +449,9 @@ def substep(state, params, quirks="reference", svd_gauge="consistent", return_gr
     if dim == 3:
         grid_op_3d(res, params["dx"], params["dt"], params["gravity"], gv, gm,
                    boundary=params.get("boundary", "reference"), bound=params.get("bound", 3))
+        if params.get("planes"):
+            pts, nrm, pq = params["planes"]
+            collide_planes(pts, nrm, res, params["dx"], gv, quirks=pq)
     else:
         grid_op_2d(res, params["dt"], params["gravity"], gv, gm,
                    boundary=params.get("boundary", "reference"), bound=params.get("bound", 3))

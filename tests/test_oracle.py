@@ -318,3 +318,59 @@ def test_golden_fixtures_match_oracle():
     g = np.load(os.path.join(GOLDEN, "oracle_3d_cubes.npz"))
     for k in ("x", "v", "F", "C", "Jp"):
         assert np.allclose(s[k], g[f"s1_{k}"], rtol=1e-12, atol=1e-14), k
+
+
+# ------------------------------------------------ plane colliders ----
+# three_d/grid_op.py:50-67.  Pinned by node counts derived by hand (integer
+# lattice arguments), not by re-evaluating the oracle's formula.
+@pytest.mark.parametrize("axis,sign,c,scale", [(0, 1, 0.33, 1.0), (1, 1, 0.6, 3.0), (2, -1, 0.71, 0.5),
+                                               (1, -1, 0.1, 7.0)])
+def test_collide_planes_textbook_axis_counts(axis, sign, c, scale):
+    res, dx = 16, 1.0 / 16
+    point = np.zeros(3)
+    point[axis] = c
+    normal = np.zeros(3)
+    normal[axis] = sign * scale
+    gv = np.ones((res + 1,) * 3 + (3,))
+    O.collide_planes([point], [normal], res, dx, gv, quirks="textbook")
+    # sign * (i dx - c) < 0: i < c/dx (sign +) or i > c/dx (sign -); c is off the lattice
+    rows = sum(1 for i in range(res + 1) if (i < c / dx if sign > 0 else i > c / dx))
+    assert int((gv[..., 0] == 0).sum()) == rows * (res + 1) ** 2
+    assert np.all((gv == 0).all(-1) | (gv == 1).all(-1))  # whole nodes only
+
+
+def test_collide_planes_reference_quirk_hand_counts():
+    res, dx = 4, 0.25
+    # normal (0, 0, 2): n' = (0.5, 0.5, 2.5); dot < 0  <=>  i + j + 5k < 14 -> 25 + 25 + 10 = 60 nodes
+    gv = np.ones((5, 5, 5, 3))
+    O.collide_planes([(0.5, 0.5, 0.5)], [(0, 0, 2)], res, dx, gv)
+    assert int((gv[..., 0] == 0).sum()) == 60
+    # normal (0, -2, 0): n' = (0.5, -1.5, 0.5); dot < 0  <=>  3j > i + k + 2 -> 58 nodes
+    gv = np.ones((5, 5, 5, 3))
+    O.collide_planes([(0.5, 0.5, 0.5)], [(0, -2, 0)], res, dx, gv)
+    assert int((gv[..., 0] == 0).sum()) == 58
+    # normal (0, -1, 0): n' = (1, 0, 1); dot = (i + k) dx >= 0 everywhere -> nothing is zeroed
+    gv = np.ones((5, 5, 5, 3))
+    O.collide_planes([(0.0, 0.5, 0.0)], [(0, -1, 0)], res, dx, gv)
+    assert int((gv == 0).sum()) == 0
+
+
+def test_collide_planes_matches_node_loop():
+    rng = np.random.default_rng(3)
+    res, dx = 8, 1.0 / 8
+    pts, nrm = rng.random((3, 3)), rng.normal(size=(3, 3))
+    for quirks in ("reference", "textbook"):
+        gv = rng.normal(size=(res + 1,) * 3 + (3,))
+        ref = gv.copy()
+        O.collide_planes(pts, nrm, res, dx, gv, quirks=quirks)
+        for p, n in zip(pts, nrm):
+            d2 = float(n[0]) * float(n[0]) + float(n[1]) * float(n[1]) + float(n[2]) * float(n[2])
+            den = d2 ** 0.5
+            nn = [float(v) + 1.0 / den if quirks == "reference" else float(v) / den for v in n]
+            for i in range(res + 1):
+                for j in range(res + 1):
+                    for k in range(res + 1):
+                        o = (i * dx - float(p[0]), j * dx - float(p[1]), k * dx - float(p[2]))
+                        if (o[0] * nn[0] + o[1] * nn[1]) + o[2] * nn[2] < 0:
+                            ref[i, j, k] = 0.0
+        assert np.array_equal(gv, ref)
