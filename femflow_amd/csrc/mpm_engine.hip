@@ -1214,12 +1214,13 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
 }
 
 int Engine::upload(const mpm_particles* p, const int32_t* scene_ids) {
-  if (!p || p->dtype != MPM_F32 || p->n < 0 || p->n > n_max || !p->x || !p->v || !p->F || !p->C ||
-      !p->Jp || !p->mass || !p->mu0 || !p->lambda0) {
+  // an empty state (n = 0) needs no arrays
+  if (!p || p->dtype != MPM_F32 || p->n < 0 || p->n > n_max ||
+      (p->n > 0 && (!p->x || !p->v || !p->F || !p->C || !p->Jp || !p->mass || !p->mu0 || !p->lambda0))) {
     set_error("mpm_state_upload: needs FP32 arrays x, v, F, C, Jp, mass, mu0, lambda0 and n <= n_max");
     return MPM_ERR_ARG;
   }
-  if (nmat > 1 && !p->material) {
+  if (nmat > 1 && p->n > 0 && !p->material) {
     set_error("mpm_state_upload: material ids are required when the table has more than one entry");
     return MPM_ERR_ARG;
   }
@@ -1236,7 +1237,7 @@ int Engine::upload(const mpm_particles* p, const int32_t* scene_ids) {
 }
 
 int Engine::download(mpm_particles* p) {
-  if (!p || p->dtype != MPM_F32 || p->n != n || !p->x || !p->v || !p->F || !p->C || !p->Jp) {
+  if (!p || p->dtype != MPM_F32 || p->n != n || (n > 0 && (!p->x || !p->v || !p->F || !p->C || !p->Jp))) {
     set_error("mpm_state_download: needs FP32 arrays x, v, F, C, Jp with the uploaded n");
     return MPM_ERR_ARG;
   }
@@ -1414,6 +1415,7 @@ int Engine::run(int nsteps, double* sums, float* frames) {
 }
 
 int Engine::positions(float* x) {
+  if (n == 0) return MPM_OK;
   if (!x) {
     set_error("mpm_state_positions: null output");
     return MPM_ERR_ARG;
@@ -1428,7 +1430,10 @@ int Engine::take_counter(int which, int64_t* out) {
   unsigned long long v = 0;
   unsigned long long* p = which == 0 ? d_oob : d_fallback;
   HIPCHK(hipMemcpy(&v, p, sizeof(v), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemset(p, 0, sizeof(v)));
+  // cleared on the engine's own stream, ahead of its later work
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, own_stream, reinterpret_cast<uint32_t*>(p),
+                     (int64_t)2, 0u);
+  HIPCHK(hipStreamSynchronize(own_stream));
   *out = (int64_t)v;
   return MPM_OK;
 }
@@ -1437,8 +1442,9 @@ int Engine::particle_sums(double* out) {
   double* d = nullptr;
   int rc;
   if ((rc = alloc((void**)&d, sizeof(double) * 8))) return rc;
-  HIPCHK(hipStreamSynchronize(own_stream));
-  HIPCHK(hipMemset(d, 0, sizeof(double) * 8));
+  // cleared on the engine's own stream (a null-stream memset would not be
+  // ordered with it)
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(1), dim3(64), 0, own_stream, d, (int64_t)8, 0.0);
   if (n > 0)
     hipLaunchKernelGGL(particle_sums_kernel, dim3(256), dim3(ENG_THREADS), 0, own_stream, st[cur], n, d);
   HIPCHK(hipStreamSynchronize(own_stream));

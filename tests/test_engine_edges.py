@@ -1,0 +1,83 @@
+"""Engine edge cases against the oracle: an empty engine, thousands of
+particles sharing one cell (many chunks per block, the fixed-point bound
+under heavy overlap), and particles within half a cell of the low walls,
+where the reference's truncated base gives fx < 0.5 and a negative middle
+weight (three_d/p2g.py:50-55, quirk Q1)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mpm_oracle as O
+from tests._parity import field_errors
+from tests.gpu_helpers import engine_for, engine_roundtrip, oracle_copy_from, to_torch_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def test_engine_empty_state(gpu):
+    st, pr = O.scene_3d_cubes(n=64, res=32)
+    eng = engine_for(st, pr, gpu, sort_every=4)
+    z3 = torch.zeros((0, 3), device=gpu)
+    z33 = torch.zeros((0, 3, 3), device=gpu)
+    z = torch.zeros((0,), device=gpu)
+    eng.upload(z3, z3, z33, z33, z, z, z, z, torch.zeros((0,), dtype=torch.int32, device=gpu))
+    sums = torch.zeros((6, 8), dtype=torch.float64, device=gpu)
+    eng.run(6, sums=sums)
+    eng.download(z3.clone(), z3.clone(), z33.clone(), z33.clone(), z.clone())
+    assert eng.oob_count() == 0
+    assert float(sums.abs().sum()) == 0.0
+    assert eng.positions().shape == (0, 3)
+    eng.close()
+
+
+@pytest.mark.parametrize("chunk", [256, 512])
+def test_engine_thousands_in_one_cell(gpu, chunk):
+    res = 32
+    elastic = (O.MATERIALS_C2[0],)
+    # 6000 particles in one cell plus a sparse cube around it
+    st, pr = O.scene_3d_cubes(n=6000, res=res, seed=5, side=1.0 / res, corners=((0.5, 0.5, 0.5),),
+                              materials=elastic, vel_sigma=0.3)
+    st2, _ = O.scene_3d_cubes(n=1500, res=res, seed=6, side=0.15, corners=((0.44, 0.44, 0.44),),
+                              materials=elastic, vel_sigma=0.3)
+    for k in st:
+        st[k] = np.concatenate([st[k], st2[k]])
+    ts = to_torch_state(st, gpu)
+    ref = oracle_copy_from(ts, st)
+    eng = engine_for(st, pr, gpu, sort_every=2, chunk=chunk, use_graph=True)
+    nsteps = 4
+    out = engine_roundtrip(eng, ts, nsteps)
+    for _ in range(nsteps):
+        O.substep(ref, pr)
+    errs = field_errors(out, ref, pr["inv_dx"])
+    _say("one cell", chunk, errs)
+    for k in ("x", "v", "F"):
+        assert errs[k] < 1e-4, errs
+    assert errs["Jp"] < 1e-4, errs
+    eng.close()
+
+
+def test_engine_low_wall_truncated_base(gpu):
+    res = 32
+    dx = 1.0 / res
+    elastic = (O.MATERIALS_C2[0],)
+    # a slab hugging the x = 0 and z = 0 walls: x, z in [0, 1.5 dx)
+    st, pr = O.scene_3d_cubes(n=4000, res=res, seed=9, side=1.5 * dx, corners=((0.0, 0.5, 0.0),),
+                              materials=elastic, vel_sigma=0.0)
+    st["x"][:, 1] = 0.5 + 0.2 * np.random.default_rng(1).random(len(st["x"]))
+    fx0 = st["x"][:, 0] * res
+    assert (fx0 < 0.134).sum() > 100  # middle weight negative for these (w1 = 0.75 - (fx - 1)^2)
+    ts = to_torch_state(st, gpu)
+    ref = oracle_copy_from(ts, st)
+    eng = engine_for(st, pr, gpu, sort_every=1)
+    out = engine_roundtrip(eng, ts, 1)
+    O.substep(ref, pr)
+    errs = field_errors(out, ref, pr["inv_dx"])
+    _say("low wall", errs)
+    for k in ("x", "v", "F"):
+        assert errs[k] < 1e-5, errs
+    assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
+    eng.close()
