@@ -160,7 +160,23 @@ def rollout(x0: torch.Tensor, v0: torch.Tensor, mu0: torch.Tensor, lam0: torch.T
         F0 = torch.eye(3, device=x0.device, dtype=torch.float32).expand(n, 3, 3)
     if C0 is None:
         C0 = torch.zeros((n, 3, 3), device=x0.device, dtype=torch.float32)
-    return _Rollout.apply(x0, v0, F0, C0, mu0, lam0, mass, cfg)
+    # The scatters aggregate particles of one stencil base within a wave, so
+    # the rollout runs in cell order; results (and gradients, through the
+    # indexing) come back in the caller's order.
+    perm = cell_order(x0.detach(), cfg.res)
+    inv = torch.empty_like(perm)
+    inv[perm] = torch.arange(n, device=perm.device)
+    pick = lambda t: t[perm] if t.numel() > 1 else t  # noqa: E731
+    xT, vT = _Rollout.apply(x0[perm], v0[perm], F0[perm], C0[perm], pick(mu0), pick(lam0), pick(mass), cfg)
+    return xT[inv], vT[inv]
+
+
+def cell_order(x: torch.Tensor, res: int) -> torch.Tensor:
+    """Stable order of particles by stencil base (row-major cell index of
+    trunc(x res - 0.5), three_d/p2g.py:50)."""
+    b = torch.clamp((x.double() * res - 0.5).trunc().long(), 0, res)
+    key = (b[:, 0] * (res + 1) + b[:, 1]) * (res + 1) + b[:, 2]
+    return torch.argsort(key, stable=True)
 
 
 def lame_from_E(E: torch.Tensor, nu: float):

@@ -71,13 +71,22 @@ __device__ __forceinline__ void bspline_d(float fx, float w[3], float dw[3]) {
 
 // ---------------------------------------------------------------- G2P^T ----
 __global__ __launch_bounds__(128) void adj_g2p_kernel(AdjArgs a) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
+  // lanes past the end or out of the grid stay in the wave (its segmented
+  // atomics need every lane) and write nothing
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool inrange = p0 < a.n;
+  const int64_t p = inrange ? p0 : a.n - 1;
   int base[3];
   float fx[3];
-  if (!adj_locate(a.x + 3 * p, a.inv_dx, a.res, base, fx)) {
-    if (a.oob) atomicAdd(a.oob, 1ull);
-    return;
+  const bool located = adj_locate(a.x + 3 * p, a.inv_dx, a.res, base, fx);
+  if (inrange && !located && a.oob) atomicAdd(a.oob, 1ull);
+  const bool valid = inrange && located;
+  if (!located) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      base[d] = 0;
+      fx[d] = 1.f;
+    }
   }
   float w[3][3], dw[3][3];
 #pragma unroll
@@ -140,6 +149,7 @@ __global__ __launch_bounds__(128) void adj_g2p_kernel(AdjArgs a) {
   float gx[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) gx[c] = -c4 * (Ct[c] * vn[0] + Ct[3 + c] * vn[1] + Ct[6 + c] * vn[2]);
+  const WaveSeg sg = wave_segments(valid, (int)((base[0] * r1 + base[1]) * r1 + base[2]));
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -155,13 +165,17 @@ __global__ __launch_bounds__(128) void adj_g2p_kernel(AdjArgs a) {
           gb[d] = vt[d] + c4 * (Ct[3 * d] * dp[0] + Ct[3 * d + 1] * dp[1] + Ct[3 * d + 2] * dp[2]);
         float* dst = a.gbar + 4 * node;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) atomicAdd(dst + d, wt * gb[d]);
+        for (int d = 0; d < 3; ++d) {
+          const float r = seg_reduce(wt * gb[d], sg);
+          if (sg.tail) atomicAdd(dst + d, r);
+        }
         // weight derivative
         const float s = g[0] * gb[0] + g[1] * gb[1] + g[2] * gb[2];
         gx[0] += dw[0][i] * w[1][j] * w[2][k] * s;
         gx[1] += w[0][i] * dw[1][j] * w[2][k] * s;
         gx[2] += w[0][i] * w[1][j] * dw[2][k] * s;
       }
+  if (!valid) return;
 #pragma unroll
   for (int d = 0; d < 3; ++d) a.xb[3 * p + d] = xo[d] + a.inv_dx * gx[d];
 #pragma unroll

@@ -428,6 +428,49 @@ template <typename T> MPM_DEV void svd2(const T* F, T U[4], T s[2], T V[4]) {
 }
 
 // -------------------------------------------------------------------------
+// Segmented wave aggregation for particle-to-node atomic scatters.  Lanes on
+// consecutive positions whose particles share a stencil base (key) form a
+// segment; seg_reduce leaves the segment's sum in its last lane, which alone
+// adds it to memory.  Particles in random order give one-lane segments (no
+// change); particles ordered by cell cut the atomics by up to the wave width,
+// which matters on this part, where same-address device atomics serialise.
+// Every lane of the wave must call both (no early return before them).
+struct WaveSeg {
+  int start;  // first lane of this lane's segment
+  int steps;  // doubling steps that cover the longest segment (wave-uniform)
+  bool tail;  // last lane of its segment and holds a particle
+};
+
+MPM_DEV WaveSeg wave_segments(bool valid, int key) {
+  const int lane = (int)__lane_id();
+  const int kprev = __shfl_up(key, 1);
+  const int vprev = __shfl_up(valid ? 1 : 0, 1);
+  const bool head = lane == 0 || !valid || !vprev || kprev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = heads & ((2ull << lane) - 1ull);  // heads at lanes <= this one
+  WaveSeg sg;
+  sg.start = 63 - __clzll((long long)upto);
+  sg.tail = valid && (lane == 63 || ((heads >> (lane + 1)) & 1ull));
+  int len = lane - sg.start + 1;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) len = max(len, __shfl_xor(len, o));
+  int steps = 0;
+  while ((1 << steps) < len) ++steps;
+  sg.steps = steps;
+  return sg;
+}
+
+template <typename T> MPM_DEV T seg_reduce(T v, const WaveSeg& sg) {
+  const int lane = (int)__lane_id();
+  for (int i = 0; i < sg.steps; ++i) {
+    const int off = 1 << i;
+    const T u = __shfl_up(v, (unsigned)off);
+    if (lane - off >= sg.start) v += u;
+  }
+  return v;
+}
+
+// -------------------------------------------------------------------------
 // Sticky half-space collider of three_d/grid_op.py:50-67
 // (check_collision_points): every node I of the plane's scene with
 // dot(I dx - p, n) < 0 gets zero velocity.  n holds the reference's

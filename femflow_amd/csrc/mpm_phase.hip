@@ -65,13 +65,22 @@ __device__ __forceinline__ bool locate(const T* xp, T inv_dx, int res, int base[
 
 template <typename T, int DIM>
 __global__ void phase_p2g_kernel(PhaseArgs<T> a) {
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
+  // lanes past the end or out of the grid stay in the wave (its segmented
+  // atomics need every lane) with a zero contribution
+  const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool inrange = p0 < a.n;
+  const int64_t p = inrange ? p0 : a.n - 1;
   int base[DIM];
   T fx[DIM];
-  if (!locate<T, DIM>(a.x + p * DIM, a.inv_dx, a.res, base, fx)) {
-    if (a.oob) atomicAdd(a.oob, 1ull);
-    return;
+  const bool located = locate<T, DIM>(a.x + p * DIM, a.inv_dx, a.res, base, fx);
+  if (inrange && !located && a.oob) atomicAdd(a.oob, 1ull);
+  const bool valid = inrange && located;
+  if (!located) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      base[d] = 0;
+      fx[d] = T(1);
+    }
   }
   T w[DIM][3];
 #pragma unroll
@@ -80,14 +89,19 @@ __global__ void phase_p2g_kernel(PhaseArgs<T> a) {
   T h, tc, ts;
   material_of(a, p, kind, h, tc, ts);
   const int64_t mi = a.mat_scalar ? 0 : p;
-  const T m = a.mass[mi];
+  const T m = valid ? a.mass[mi] : T(0);
   T mu, lam;
   lame<T>(kind, h, a.mu0[mi], a.lam0[mi], a.Jp[p], mu, lam);
   const bool textbook = a.quirks == MPM_QUIRKS_TEXTBOOK;
   const int64_t r1 = a.res + 1;
   if constexpr (DIM == 3) {
+    const WaveSeg sg = wave_segments(valid, (int)((base[0] * r1 + base[1]) * r1 + base[2]));
     T A[9];
     affine3<T>(a.F + 9 * p, a.C + 9 * p, m, mu, lam, kind, a.inv_dx, a.dt, a.vol, textbook, A);
+    if (!valid) {
+#pragma unroll
+      for (int e = 0; e < 9; ++e) A[e] = T(0);
+    }
     const T mv[3] = {a.v[3 * p] * m, a.v[3 * p + 1] * m, a.v[3 * p + 2] * m};
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -101,13 +115,20 @@ __global__ void phase_p2g_kernel(PhaseArgs<T> a) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             const T ad = A[3 * d] * dp[0] + A[3 * d + 1] * dp[1] + A[3 * d + 2] * dp[2];
-            atomicAdd(a.grid_v + 3 * node + d, weight * (mv[d] + ad));
+            const T r = seg_reduce(weight * (mv[d] + ad), sg);
+            if (sg.tail) atomicAdd(a.grid_v + 3 * node + d, r);
           }
-          atomicAdd(a.grid_m + node, weight * m);
+          const T rm = seg_reduce(weight * m, sg);
+          if (sg.tail) atomicAdd(a.grid_m + node, rm);
         }
   } else {
+    const WaveSeg sg = wave_segments(valid, (int)(base[0] * r1 + base[1]));
     T A[4];
     affine2<T>(a.F + 4 * p, a.C + 4 * p, m, mu, lam, kind, a.inv_dx, a.dt, a.vol, textbook, A);
+    if (!valid) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) A[e] = T(0);
+    }
     const T mv[2] = {a.v[2 * p] * m, a.v[2 * p + 1] * m};
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -119,9 +140,11 @@ __global__ void phase_p2g_kernel(PhaseArgs<T> a) {
 #pragma unroll
         for (int d = 0; d < 2; ++d) {
           const T ad = A[2 * d] * dp[0] + A[2 * d + 1] * dp[1];
-          atomicAdd(a.grid_v + 2 * node + d, weight * (mv[d] + ad));
+          const T r = seg_reduce(weight * (mv[d] + ad), sg);
+          if (sg.tail) atomicAdd(a.grid_v + 2 * node + d, r);
         }
-        atomicAdd(a.grid_m + node, weight * m);
+        const T rm = seg_reduce(weight * m, sg);
+        if (sg.tail) atomicAdd(a.grid_m + node, rm);
       }
   }
 }
