@@ -1,14 +1,15 @@
 """Differentiable rollouts of the explicit substep (hand-written adjoint).
 
 ``rollout(x0, v0, mu0, lam0, mass, cfg)`` runs ``cfg.nsteps`` elastic
-(fixed-corotated) 3D substeps of mls_mpm.py:40-79 on a dense (res+1)^3 grid
-in FP32 on the GPU and returns the final positions and velocities.  It is a
+(fixed-corotated) 3D substeps of mls_mpm.py:40-79 in FP32 on the GPU and
+returns the final positions and velocities.  It is a
 ``torch.autograd.Function``: the backward pass is the library's adjoint of
-one substep (``mpm_adjoint_step``), applied in reverse order.  The forward
-keeps a checkpoint of the particle state every ``cfg.checkpoint_every``
-substeps; the backward replays one checkpoint interval at a time, keeping
-that interval's per-substep states (24 floats per particle per substep), and
-reverses it.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
+one substep (``mpm_adjoint_step``, on dense (res+1)^3 grids), applied in
+reverse order.  The forward substeps run on the particle-resident engine and
+keep a checkpoint of the particle state every ``cfg.checkpoint_every``
+substeps; the backward replays one checkpoint interval at a time on the
+engine, keeping that interval's per-substep states (24 floats per particle
+per substep), and reverses it.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
 or scalar); mass is a constant.
 
 The reference has no gradients; the forward it differentiates is its step
@@ -98,6 +99,48 @@ class DenseStepper:
             raise RuntimeError(f"{n} particle stencil(s) left the grid")  # three_d/p2g.py:51-52
 
 
+class EngineStepper:
+    """Forward substeps on the particle-resident engine (the fast path: binned
+    particles, LDS arenas); the adjoint of each substep runs on the dense
+    grids of DenseStepper from the states this stepper produces."""
+
+    def __init__(self, cfg: RolloutConfig, n: int, device, mass, mu0, lam0):
+        from .engine import Engine, MaterialSpec, SceneSpec
+
+        self.device = torch.device(device)
+        self.n = int(n)
+        self.eng = Engine([SceneSpec(res=cfg.res, dt=cfg.dt, gravity=cfg.gravity, volume=cfg.volume,
+                                     boundary=cfg.boundary, bound=cfg.bound, quirks=cfg.quirks)],
+                          [MaterialSpec("elastic", cfg.hardening)], n_max=max(self.n, 1), device=self.device)
+        self.eng.tune(sort_every=16)
+        full = lambda t: (t.expand(self.n) if t.numel() == 1 else t.reshape(self.n)).contiguous()  # noqa: E731
+        self.mass, self.mu0, self.lam0 = full(mass), full(mu0), full(lam0)
+        self.Jp = torch.ones(self.n, dtype=torch.float32, device=self.device)
+        self.kind = torch.zeros(self.n, dtype=torch.int32, device=self.device)
+
+    def load(self, x, v, F, C):
+        self.eng.upload(x.contiguous(), v.contiguous(), F.contiguous(), C.contiguous(), self.Jp, self.mass,
+                        self.mu0, self.lam0, self.kind)
+
+    def run(self, nsteps: int):
+        self.eng.run(nsteps, check_oob=False)
+
+    def state(self):
+        n, d = self.n, self.device
+        x, v = torch.empty((n, 3), device=d), torch.empty((n, 3), device=d)
+        F, C = torch.empty((n, 3, 3), device=d), torch.empty((n, 3, 3), device=d)
+        self.eng.download(x, v, F, C, torch.empty(n, device=d))
+        return x, v, F, C
+
+    def check_oob(self):
+        n = self.eng.oob_count()
+        if n:
+            raise RuntimeError(f"{n} particle stencil(s) left the grid")  # three_d/p2g.py:51-52
+
+    def close(self):
+        self.eng.close()
+
+
 class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x0, v0, F0, C0, mu0, lam0, mass, cfg: RolloutConfig):
@@ -105,16 +148,18 @@ class _Rollout(torch.autograd.Function):
         n = x0.shape[0]
         st = DenseStepper(cfg, n, dev)
         mass_, mu_, lam_ = _f32(mass, dev), _f32(mu0, dev), _f32(lam0, dev)
+        fwd = EngineStepper(cfg, n, dev, mass_, mu_, lam_)
         x, v = _f32(x0, dev).clone(), _f32(v0, dev).clone()
         F, C = _f32(F0, dev).clone(), _f32(C0, dev).clone()
         K = max(1, int(cfg.checkpoint_every))
         ckpts = []
-        for s in range(cfg.nsteps):
-            if s % K == 0:
-                ckpts.append((x.clone(), v.clone(), F.clone(), C.clone()))
-            st.step(x, v, F, C, mass_, mu_, lam_)
-        st.check_oob()
-        ctx.stepper, ctx.ckpts, ctx.cfg = st, ckpts, cfg
+        fwd.load(x, v, F, C)
+        for s0 in range(0, cfg.nsteps, K):
+            ckpts.append((x, v, F, C) if s0 == 0 else fwd.state())
+            fwd.run(min(K, cfg.nsteps - s0))
+        x, v, _, _ = fwd.state()
+        fwd.check_oob()
+        ctx.stepper, ctx.fwd, ctx.ckpts, ctx.cfg = st, fwd, ckpts, cfg
         ctx.mats = (mass_, mu_, lam_)
         ctx.shapes = (mu0.shape, lam0.shape, mu0.dtype, lam0.dtype)
         return x, v
@@ -132,18 +177,24 @@ class _Rollout(torch.autograd.Function):
         mub = torch.zeros_like(mu_)
         lamb = torch.zeros_like(lam_)
         K = max(1, int(cfg.checkpoint_every))
+        fwd = ctx.fwd
         for seg in range(len(ctx.ckpts) - 1, -1, -1):
             s0 = seg * K
             s1 = min(s0 + K, cfg.nsteps)
-            x, v, F, C = (t.clone() for t in ctx.ckpts[seg])
+            # recompute the interval's per-substep input states on the engine
+            fwd.load(*ctx.ckpts[seg])
             states = []
-            for _ in range(s0, s1):
-                states.append((x.clone(), v.clone(), F.clone(), C.clone()))
-                st.step(x, v, F, C, mass_, mu_, lam_)
+            for s in range(s0, s1):
+                states.append(ctx.ckpts[seg] if s == s0 else fwd.state())
+                if s + 1 < s1:
+                    fwd.run(1)
             for (x, v, F, C) in reversed(states):
                 st.adjoint(x, v, F, C, mass_, mu_, lam_, xb, vb, Fb, Cb, mub, lamb)
             del states
+        fwd.check_oob()
         st.check_oob()
+        fwd.close()
+        ctx.fwd = None
         mu_shape, lam_shape, mu_dt, lam_dt = ctx.shapes
         ctx.ckpts = None
         return (xb, vb, Fb, Cb, mub.reshape(mu_shape).to(mu_dt), lamb.reshape(lam_shape).to(lam_dt), None, None)
