@@ -40,6 +40,13 @@ def test_argument_errors_do_not_touch_the_gpu():
     assert L.mpm_ctx_tune(None, 512, 1, 1, 1) == _lib.MPM_ERR_ARG
     bad = _lib.Params(4, 16, 16.0, 1 / 16, 1e-4, 1.0, -9.8, 0, 3, 0, 1)
     assert L.mpm_grid_op(None, ct.byref(bad), _lib.MPM_F32, None, None) == _lib.MPM_ERR_ARG
+    # plane colliders: null grid, too many planes, bad quirks mode, null context
+    pts = (ct.c_double * 3)(0.5, 0.5, 0.5)
+    nrm = (ct.c_double * 3)(0.0, 1.0, 0.0)
+    assert L.mpm_collide_planes(None, _lib.MPM_F32, 8, 0.125, 1, pts, nrm, 0, None) == _lib.MPM_ERR_ARG
+    assert L.mpm_collide_planes(None, _lib.MPM_F32, 8, 0.125, 17, pts, nrm, 0, ct.c_void_p(16)) == _lib.MPM_ERR_ARG
+    assert L.mpm_collide_planes(None, _lib.MPM_F32, 8, 0.125, 1, pts, nrm, 7, ct.c_void_p(16)) == _lib.MPM_ERR_ARG
+    assert L.mpm_ctx_set_planes(None, 0, 1, pts, nrm, 0) == _lib.MPM_ERR_ARG
     with pytest.raises(ValueError):
         _lib.check(_lib.MPM_ERR_ARG)
     with pytest.raises(RuntimeError):
