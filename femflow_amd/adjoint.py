@@ -125,6 +125,14 @@ class EngineStepper:
     def run(self, nsteps: int):
         self.eng.run(nsteps, check_oob=False)
 
+    def replay(self, nsteps: int):
+        """States after each of the next ``nsteps`` substeps (one engine run)."""
+        n, d = self.n, self.device
+        x, v = torch.empty((nsteps, n, 3), device=d), torch.empty((nsteps, n, 3), device=d)
+        F, C = torch.empty((nsteps, n, 3, 3), device=d), torch.empty((nsteps, n, 3, 3), device=d)
+        self.eng.run_states(nsteps, x, v, F, C, torch.empty((nsteps, n), device=d), check_oob=False)
+        return [(x[k], v[k], F[k], C[k]) for k in range(nsteps)]
+
     def state(self):
         n, d = self.n, self.device
         x, v = torch.empty((n, 3), device=d), torch.empty((n, 3), device=d)
@@ -183,11 +191,7 @@ class _Rollout(torch.autograd.Function):
             s1 = min(s0 + K, cfg.nsteps)
             # recompute the interval's per-substep input states on the engine
             fwd.load(*ctx.ckpts[seg])
-            states = []
-            for s in range(s0, s1):
-                states.append(ctx.ckpts[seg] if s == s0 else fwd.state())
-                if s + 1 < s1:
-                    fwd.run(1)
+            states = [ctx.ckpts[seg]] + (fwd.replay(s1 - s0 - 1) if s1 - s0 > 1 else [])
             for (x, v, F, C) in reversed(states):
                 st.adjoint(x, v, F, C, mass_, mu_, lam_, xb, vb, Fb, Cb, mub, lamb)
             del states

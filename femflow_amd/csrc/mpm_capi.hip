@@ -404,6 +404,15 @@ int32_t mpm_run_frames(mpm_ctx* ctx, int32_t nsteps, float* frames) {
   return e.run(nsteps, nullptr, frames);
 }
 
+int32_t mpm_run_states(mpm_ctx* ctx, int32_t nsteps, float* x, float* v, float* F, float* C, float* Jp) {
+  if (!ctx || (nsteps > 0 && ctx->eng.n > 0 && (!x || !v || !F || !C || !Jp))) {
+    set_error("mpm_run_states: null argument");
+    return MPM_ERR_ARG;
+  }
+  const mpm::StateSink sink{x, v, F, C, Jp};
+  return ctx->eng.run(nsteps, nullptr, nullptr, &sink);
+}
+
 int32_t mpm_state_positions(mpm_ctx* ctx, float* x) {
   if (!ctx) {
     set_error("mpm_state_positions: null context");

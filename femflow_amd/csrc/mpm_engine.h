@@ -30,6 +30,12 @@ struct CorrExtract {
   int64_t max_rows;
 };
 
+// Per-substep state output (caller order): slot k of each array receives the
+// state after substep k.
+struct StateSink {
+  float *x, *v, *F, *C, *Jp;
+};
+
 struct TimedLaunch {
   int kind;  // 0 P2G, 1 G2P2G, 2 G2P, 3 grid update
   hipEvent_t e0, e1;
@@ -93,8 +99,9 @@ struct Engine {
   int download(mpm_particles* p);
   void enqueue_bin(hipStream_t s);
   void enqueue_segment(int len, hipStream_t s, double* sums, int max_steps, float* frames = nullptr,
-                       int64_t step0 = 0);
+                       int64_t step0 = 0, const StateSink* sink = nullptr);
   void enqueue_frame(hipStream_t s, float* frames, int64_t step);
+  void enqueue_state(hipStream_t s, const StateSink* sink, int64_t step);
   int positions(float* x);
   void launch_transfer(int mode, hipStream_t s, const EngineArgs& a);
   void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k);
@@ -108,7 +115,7 @@ struct Engine {
   int grid_velocity(const uint32_t* nodes, int64_t count, float* out);
   int set_planes(int scene, int count, const double* points, const double* normals, int quirks);
   hipEvent_t tevent();
-  int run(int nsteps, double* sums, float* frames = nullptr);
+  int run(int nsteps, double* sums, float* frames = nullptr, const StateSink* sink = nullptr);
   int take_counter(int which, int64_t* out);
   int particle_sums(double* out);
   int kernel_times(double* ms, int64_t* counts);

@@ -1346,8 +1346,15 @@ void Engine::fill_u32(hipStream_t s, uint32_t* p, int64_t count, uint32_t v) {
   hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(count, 256)), dim3(256), 0, s, p, count, v);
 }
 
+void Engine::enqueue_state(hipStream_t s, const StateSink* sink, int64_t step) {
+  if (!sink || n <= 0) return;
+  const size_t o = (size_t)step * (size_t)n;
+  hipLaunchKernelGGL(download_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, st[cur], n, sink->x + 3 * o,
+                     sink->v + 3 * o, sink->F + 9 * o, sink->C + 9 * o, sink->Jp + o);
+}
+
 void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps, float* frames,
-                             int64_t step0) {
+                             int64_t step0, const StateSink* sink) {
   const int ws = sums ? 1 : 0;
   hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws);
   enqueue_bin(s);
@@ -1359,15 +1366,17 @@ void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps
   for (int k = 1; k < len; ++k) {
     launch_transfer(1, s, args(sums, max_steps, k & 1));
     enqueue_frame(s, frames, step0 + k - 1);  // substep k-1 finished its G2P
+    enqueue_state(s, sink, step0 + k - 1);
     if (corr_on) launch_cellf(s, args(sums, max_steps, k & 1));
     launch_grid(s, args(sums, max_steps, k & 1), ws, len, k);
     if (corr_on) launch_corrector(s, args(sums, max_steps, k & 1), nullptr);
   }
   launch_transfer(2, s, args(sums, max_steps, len & 1));
   enqueue_frame(s, frames, step0 + len - 1);
+  enqueue_state(s, sink, step0 + len - 1);
 }
 
-int Engine::run(int nsteps, double* sums, float* frames) {
+int Engine::run(int nsteps, double* sums, float* frames, const StateSink* sink) {
   if (nsteps < 0) {
     set_error("mpm_run: nsteps must be >= 0");
     return MPM_ERR_ARG;
@@ -1383,7 +1392,7 @@ int Engine::run(int nsteps, double* sums, float* frames) {
   int done = 0;
   while (done < nsteps) {
     const int len = (nsteps - done) < sort_every ? (nsteps - done) : sort_every;
-    const bool graphed = use_graph && !timing && n > 0 && !frames;
+    const bool graphed = use_graph && !timing && n > 0 && !frames && !sink;
     if (graphed) {
       const int parity = cur;
       const GraphKey key{len, parity, (uintptr_t)sums, sums ? nsteps : 0};
@@ -1404,7 +1413,7 @@ int Engine::run(int nsteps, double* sums, float* frames) {
       HIPCHK(hipGraphLaunch(exec, own_stream));
       cur = parity ^ 1;
     } else {
-      enqueue_segment(len, own_stream, sums, nsteps, frames, done);
+      enqueue_segment(len, own_stream, sums, nsteps, frames, done, sink);
     }
     HIPCHK(hipGetLastError());
     done += len;

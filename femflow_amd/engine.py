@@ -143,6 +143,24 @@ class Engine:
             if n:
                 raise RuntimeError(f"{n} particle stencil(s) left the grid")
 
+    def run_states(self, nsteps: int, x, v, F, C, Jp, check_oob: bool = True):
+        """Advance ``nsteps`` substeps, writing the full state after substep k
+        (caller order) to slot k: x, v (nsteps, N, 3), F, C (nsteps, N, 3, 3),
+        Jp (nsteps, N); float32 CUDA tensors."""
+        n = self.n
+        for t, nm, shape in ((x, "x", (nsteps, n, 3)), (v, "v", (nsteps, n, 3)), (F, "F", (nsteps, n, 3, 3)),
+                             (C, "C", (nsteps, n, 3, 3)), (Jp, "Jp", (nsteps, n))):
+            self._check_tensor(t, nm)
+            if tuple(t.shape) != shape:
+                raise ValueError(f"{nm} must have shape {shape}")
+        self._sync_stream()
+        _lib.check(self._L.mpm_run_states(self._h, int(nsteps), x.data_ptr(), v.data_ptr(), F.data_ptr(),
+                                          C.data_ptr(), Jp.data_ptr()), "run_states")
+        if check_oob:
+            k = self.oob_count()
+            if k:
+                raise RuntimeError(f"{k} particle stencil(s) left the grid")
+
     def run_frames(self, nsteps: int, frames: torch.Tensor, check_oob: bool = True):
         """Advance ``nsteps`` substeps, writing each substep's positions (caller
         order) into ``frames``, a float32 CUDA tensor (nsteps, N, 3)."""

@@ -174,6 +174,11 @@ int32_t mpm_run(mpm_ctx* ctx, int32_t nsteps, double* sums);
  * caller's order to frames (device float, nsteps * n * 3): the per-step frame
  * record of simulation.py:140-142 (map_particles_to_pos, particle.py:30-33). */
 int32_t mpm_run_frames(mpm_ctx* ctx, int32_t nsteps, float* frames);
+/* nsteps substeps writing the full particle state after substep k (caller
+ * order) to slot k: x, v (nsteps x n x 3), F, C (nsteps x n x 3 x 3), Jp
+ * (nsteps x n), FP32 device arrays.  No reference counterpart: it serves
+ * differentiable rollouts, which replay the states of a checkpoint interval. */
+int32_t mpm_run_states(mpm_ctx* ctx, int32_t nsteps, float* x, float* v, float* F, float* C, float* Jp);
 /* Current positions in the caller's order (device float, n * 3). */
 int32_t mpm_state_positions(mpm_ctx* ctx, float* x);
 /* Convenience for the reference's per-call pattern: upload, run, download. */

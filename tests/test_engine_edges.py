@@ -81,3 +81,32 @@ def test_engine_low_wall_truncated_base(gpu):
         assert errs[k] < 1e-5, errs
     assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
     eng.close()
+
+
+def test_engine_run_states_matches_frames_and_download(gpu):
+    st, pr = O.scene_3d_cubes(n=20000, res=48)
+    ts = to_torch_state(st, gpu)
+    n, k = len(st["x"]), 7
+    outs = []
+    for mode in ("states", "frames"):
+        eng = engine_for(st, pr, gpu, sort_every=3)
+        eng.upload(ts["x"], ts["v"], ts["F"], ts["C"], ts["Jp"], ts["mass"], ts["mu0"], ts["lam0"], ts["kind"])
+        if mode == "states":
+            x, v = torch.empty((k, n, 3), device=gpu), torch.empty((k, n, 3), device=gpu)
+            F, C = torch.empty((k, n, 3, 3), device=gpu), torch.empty((k, n, 3, 3), device=gpu)
+            Jp = torch.empty((k, n), device=gpu)
+            eng.run_states(k, x, v, F, C, Jp)
+            outs.append((x, v, F, C, Jp))
+        else:
+            frames = torch.empty((k, n, 3), device=gpu)
+            eng.run_frames(k, frames)
+            last = {f: torch.empty_like(ts[f]) for f in ("x", "v", "F", "C", "Jp")}
+            eng.download(last["x"], last["v"], last["F"], last["C"], last["Jp"])
+            outs.append((frames, last))
+        eng.close()
+    (x, v, F, C, Jp), (frames, last) = outs
+    torch.cuda.synchronize()
+    assert torch.equal(x, frames)
+    assert torch.equal(v[-1], last["v"]) and torch.equal(F[-1], last["F"])
+    assert torch.equal(C[-1], last["C"]) and torch.equal(Jp[-1], last["Jp"])
+    assert not torch.equal(x[0], x[-1])
