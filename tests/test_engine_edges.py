@@ -110,3 +110,45 @@ def test_engine_run_states_matches_frames_and_download(gpu):
     assert torch.equal(v[-1], last["v"]) and torch.equal(F[-1], last["F"])
     assert torch.equal(C[-1], last["C"]) and torch.equal(Jp[-1], last["Jp"])
     assert not torch.equal(x[0], x[-1])
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
+def test_engine_deferred_fp64_path(gpu):
+    """Near-degenerate deformation gradients take the deferred FP64 pass: snow
+    whose smallest singular value is below 1e-2 of the largest (FP64 SVD and
+    clamp) and elastic particles whose |det F| is below the FP32 polar guard
+    (FP64 polar).  One substep against the FP64 oracle."""
+    rng = np.random.default_rng(11)
+    mats = (O.MATERIALS_C2[0], O.MATERIALS_C2[1])
+    st, pr = O.scene_3d_cubes(n=20000, res=32, seed=12, side=0.25, corners=((0.3, 0.35, 0.3), (0.55, 0.35, 0.55)),
+                              materials=mats, vel_sigma=0.2)
+    n = len(st["x"])
+    F = st["F"].reshape(n, 3, 3)
+    picks = rng.choice(n, size=600, replace=False)
+    for p in picks:
+        if st["kind"][p] == O.SNOW:
+            s = np.diag([1.0, 0.95, 0.004])       # s_min / s_max = 0.004 < 1e-2
+        else:
+            s = np.diag([1.02, 0.98, 2e-4])       # |det F| far below 1e-3 (|F|/sqrt 3)^3
+        F[p] = _rotation(rng) @ s @ _rotation(rng)
+    st["F"] = F.reshape(st["F"].shape)
+    ts = to_torch_state(st, gpu)
+    ref = oracle_copy_from(ts, st)
+    eng = engine_for(st, pr, gpu, sort_every=1)
+    out = engine_roundtrip(eng, ts, 1)
+    fb = eng.fallback_count()
+    O.substep(ref, pr)
+    errs = field_errors(out, ref, pr["inv_dx"])
+    _say("deferred", fb, errs)
+    assert fb >= 500  # the degenerate particles went through the FP64 pass
+    for k in ("x", "v", "F"):
+        assert errs[k] < 1e-5, errs
+    assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
+    eng.close()
