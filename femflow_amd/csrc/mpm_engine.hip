@@ -315,9 +315,27 @@ __device__ __forceinline__ void contribution_bound(const float w[3][3], const P2
 
 // Accumulation arena: two 64-bit words per node (x|y and z|m channel pairs,
 // see pack_fixed2); node (a0, a1, a2) of the 8x8x8 arena lives at
-// a0*69 + a1*9 + a2, which spreads the 64 cells of a block over the LDS
-// banks (a plain 64/8/1 layout puts them on a quarter of the banks).
-constexpr int ACC_S0 = 69, ACC_S1 = 9, ACC_NODES = 7 * ACC_S0 + 7 * ACC_S1 + 7 + 1;
+// a0*101 + a1*9 + a2, which spreads the 64 cells of a block over the LDS
+// banks (a plain 64/8/1 layout puts them on a quarter of the banks).  The
+// map must be one-to-one over all 512 nodes, not only over the 6x6x6 nodes
+// that freshly binned particles reach: particles that drifted across a block
+// face since binning reach layers 0 and 7, so the row stride must be at least
+// 7*9+8 = 71 (with 69, node (e, 7, 6) shared a word with (e+1, 0, 0)).  101
+// is the smallest such stride that is 69 mod 32, i.e. keeps 69's bank pattern.
+constexpr int ACC_S0 = 101, ACC_S1 = 9, ACC_NODES = 7 * ACC_S0 + 7 * ACC_S1 + 7 + 1;
+
+constexpr bool acc_layout_one_to_one() {
+  bool used[ACC_NODES] = {};
+  for (int e0 = 0; e0 < 8; ++e0)
+    for (int e1 = 0; e1 < 8; ++e1)
+      for (int e2 = 0; e2 < 8; ++e2) {
+        const int w = e0 * ACC_S0 + e1 * ACC_S1 + e2;
+        if (w < 0 || w >= ACC_NODES || used[w]) return false;
+        used[w] = true;
+      }
+  return true;
+}
+static_assert(acc_layout_one_to_one(), "accumulation arena layout must be one-to-one on 8x8x8 nodes");
 
 // Two int32 fixed-point values in one 64-bit word whose 64-bit sum carries
 // both sums exactly: word = hi * 2^32 + lo (lo sign-extended).  Valid while

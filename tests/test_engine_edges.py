@@ -55,8 +55,8 @@ def test_engine_thousands_in_one_cell(gpu, chunk):
     errs = field_errors(out, ref, pr["inv_dx"])
     _say("one cell", chunk, errs)
     for k in ("x", "v", "F"):
-        assert errs[k] < 1e-4, errs
-    assert errs["Jp"] < 1e-4, errs
+        assert errs[k] < 1e-5, errs
+    assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
     eng.close()
 
 
@@ -148,6 +148,37 @@ def test_engine_deferred_fp64_path(gpu):
     errs = field_errors(out, ref, pr["inv_dx"])
     _say("deferred", fb, errs)
     assert fb >= 500  # the degenerate particles went through the FP64 pass
+    for k in ("x", "v", "F"):
+        assert errs[k] < 1e-5, errs
+    assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
+    eng.close()
+
+
+def test_engine_drift_between_rebins(gpu):
+    """Fast particles and rare rebinning: over 24 substeps without a rebin the
+    particles travel ~2 cells, so many leave the 8^3 arena of the block they
+    were binned in and take the global gather / scatter paths (and the arena's
+    eighth node layer its global atomics).  Particles that drift across a
+    block face both ways reach the arena's outer node layers 0 and 7, which
+    fresh binning never touches; an LDS layout that aliased node (e, 7, 6)
+    with (e+1, 0, 0) showed up here as a 3e-3 velocity error after 6 steps."""
+    elastic = (O.MATERIALS_C2[0],)
+    st, pr = O.scene_3d_cubes(n=20000, res=64, seed=21, side=0.15, corners=((0.42, 0.42, 0.42),),
+                              materials=elastic, vel_sigma=30.0)
+    rng = np.random.default_rng(22)
+    st["v"] = np.ascontiguousarray(st["v"] + rng.normal(0.0, 10.0, size=st["v"].shape))  # spread the speeds
+    ts = to_torch_state(st, gpu)
+    ref = oracle_copy_from(ts, st)
+    eng = engine_for(st, pr, gpu, sort_every=24, use_graph=False)
+    nsteps = 24
+    out = engine_roundtrip(eng, ts, nsteps)
+    x0 = ts["x"].double().cpu().numpy()
+    for _ in range(nsteps):
+        O.substep(ref, pr)
+    moved = np.abs(ref["x"] - x0).max(axis=1) * pr["inv_dx"]
+    errs = field_errors(out, ref, pr["inv_dx"])
+    _say("drift", float(np.mean(moved > 1.0)), errs)
+    assert np.mean(moved > 1.0) > 0.2  # a fifth of the particles crossed more than a cell
     for k in ("x", "v", "F"):
         assert errs[k] < 1e-5, errs
     assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
