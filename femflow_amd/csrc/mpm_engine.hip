@@ -348,6 +348,16 @@ __device__ __forceinline__ void unpack_fixed2(unsigned long long w, int& hi, int
   lo = (int)(uint32_t)w;
   hi = (int)(uint32_t)(w >> 32) - (lo >> 31);
 }
+// The same with a low channel that is never negative (the mass: the arena
+// path only takes particles whose weights and mass are >= 0, and its sum
+// stays below 2^31): no borrow, so packing is free.
+__device__ __forceinline__ unsigned long long pack_fixed2_nn(int hi, int lo_nonneg) {
+  return ((unsigned long long)(uint32_t)hi << 32) | (unsigned long long)(uint32_t)lo_nonneg;
+}
+__device__ __forceinline__ void unpack_fixed2_nn(unsigned long long w, int& hi, int& lo) {
+  lo = (int)(uint32_t)w;
+  hi = (int)(uint32_t)(w >> 32);
+}
 
 // round(w * q) to int32 for finite, in-range products.  A truncating
 // conversion is biased by up to one fixed-point unit per contribution toward
@@ -474,7 +484,10 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
           continue;
         }
         const int a0 = base[0] - O0, a1 = base[1] - O1, a2 = base[2] - O2;
-        bool inside = a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5;
+        // negative weights (the truncated base at a low wall, fx < 0.134) or
+        // mass would make a mass contribution negative: float path
+        bool inside = a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5 &&
+                      fminf(w[0][1], fminf(w[1][1], w[2][1])) >= 0.f && tm.m >= 0.f;
         if (inside) {
           float b[4];
           contribution_bound(w, tm, b);
@@ -574,7 +587,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
               const int c2 = round_fixed(wt, qij[2] + A2s[2] * (float)k);
               const int c3 = round_fixed(wt, ms);
               atomicAdd(&acc2[0][off], pack_fixed2(c1, c0));
-              atomicAdd(&acc2[1][off], pack_fixed2(c3, c2));
+              atomicAdd(&acc2[1][off], pack_fixed2_nn(c2, c3));
             }
           }
         }
@@ -589,7 +602,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         const int ph = e0 * ACC_S0 + e1 * ACC_S1 + e2;
         int q0, q1, q2, q3;
         unpack_fixed2(acc2[0][ph], q1, q0);
-        unpack_fixed2(acc2[1][ph], q3, q2);
+        unpack_fixed2_nn(acc2[1][ph], q2, q3);
         const float4 val = make_float4((float)((double)q0 * inv[0]), (float)((double)q1 * inv[1]),
                                        (float)((double)q2 * inv[2]), (float)((double)q3 * inv[3]));
         const bool to_stage = staged && e0 < STAGE && e1 < STAGE && e2 < STAGE;

@@ -183,3 +183,66 @@ def test_engine_drift_between_rebins(gpu):
         assert errs[k] < 1e-5, errs
     assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
     eng.close()
+
+
+def test_engine_sparse_scene_beyond_staging(gpu):
+    """About one particle per 4^3-cell block over most of a 128^3 grid: far
+    more chunks (one per occupied block) than the engine stages, which it
+    sizes at 2 ceil(n / 256) + 8192 arenas, so the chunks past that capacity
+    flush through global atomics and the grid update sums the staged
+    remainder with them.  The overlapping cubes move at different velocities
+    (the three materials interpenetrate) and the particles drift between the
+    rebins.  The scene is stiff for its particle mass (wave speed times
+    dt / dx = 0.45): FP32 rounding grows about 3x per substep here, equally in
+    the dense reference-order FP32 phases, so it runs two substeps."""
+    res = 128
+    st, pr = O.scene_3d_cubes(n=36000, res=res, seed=31, side=0.6,
+                              corners=((0.08, 0.08, 0.08), (0.32, 0.32, 0.32), (0.08, 0.32, 0.2)),
+                              vel_sigma=4.0)
+    base = np.trunc(st["x"] * res - 0.5).astype(np.int64) >> 2
+    occupied = len(np.unique(base[:, 0] * 4096 + base[:, 1] * 64 + base[:, 2]))
+    staged = 2 * ((len(st["x"]) + 255) // 256) + 8192
+    assert occupied > staged + 2000, (occupied, staged)
+    ts = to_torch_state(st, gpu)
+    ref = oracle_copy_from(ts, st)
+    eng = engine_for(st, pr, gpu, sort_every=2, use_graph=True)
+    nsteps = 2
+    out = engine_roundtrip(eng, ts, nsteps)
+    for _ in range(nsteps):
+        O.substep(ref, pr)
+    errs = field_errors(out, ref, pr["inv_dx"])
+    _say("sparse", occupied, staged, errs)
+    for k in ("x", "v", "F"):
+        assert errs[k] < 1e-5, errs
+    assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
+    eng.close()
+
+
+def test_engine_high_walls(gpu):
+    """A slab against the high x and z walls: stencils reach node res, the
+    last grid node, which sits in a partly filled 4^3 block (res + 1 nodes per
+    axis), and the walls' two-node layer zeroes the normal velocity
+    (three_d/grid_op.py:38-47); particles moving outward pile up there."""
+    res = 32
+    dx = 1.0 / res
+    elastic = (O.MATERIALS_C2[0],)
+    st, pr = O.scene_3d_cubes(n=4000, res=res, seed=12, side=1.7 * dx,
+                              corners=((1.0 - 2.35 * dx, 0.5, 1.0 - 2.35 * dx),), materials=elastic,
+                              vel_sigma=0.0)
+    st["x"][:, 1] = 0.5 + 0.2 * np.random.default_rng(2).random(len(st["x"]))
+    st["v"] = np.ascontiguousarray(np.tile([3.0, 0.0, 2.0], (len(st["x"]), 1)))  # toward the walls
+    t = st["x"] * res - 0.5
+    assert t[:, [0, 2]].max() < res - 1 and (np.trunc(t[:, 0]) == res - 2).sum() > 1000
+    ts = to_torch_state(st, gpu)
+    ref = oracle_copy_from(ts, st)
+    eng = engine_for(st, pr, gpu, sort_every=2)
+    nsteps = 3
+    out = engine_roundtrip(eng, ts, nsteps)
+    for _ in range(nsteps):
+        O.substep(ref, pr)
+    errs = field_errors(out, ref, pr["inv_dx"])
+    _say("high walls", errs)
+    for k in ("x", "v", "F"):
+        assert errs[k] < 1e-5, errs
+    assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
+    eng.close()
