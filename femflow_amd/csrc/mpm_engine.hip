@@ -757,10 +757,19 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
   const int t = threadIdx.x;
   const int wv = t >> 6, inner = t & 63;
   double s[7] = {0, 0, 0, 0, 0, 0, 0};
-  // One active block per wave, one node per lane.
-  for (uint32_t b = blockIdx.x * 4u + (uint32_t)wv; b < count; b += gridDim.x * 4u) {
-    const uint32_t g = a.list[b];
-    const int sid = a.block_scene[g];
+  // One active block per wave, one node per lane.  The kernel is a chain of
+  // dependent loads (list entry, neighbour chunk slots, staged arenas), so
+  // the first list entry is read together with the count (every slot below
+  // list_cap is addressable; entries past the count are ignored), and a
+  // single-scene engine skips the block -> scene lookup.
+  const bool one_scene = a.nscenes == 1;
+  const uint32_t stride = gridDim.x * 4u;
+  uint32_t b = blockIdx.x * 4u + (uint32_t)wv;
+  uint32_t g_next = b < a.list_cap ? a.list[b] : 0u;
+  for (; b < count; b += stride) {
+    const uint32_t g = g_next;
+    g_next = b + stride < a.list_cap ? a.list[b + stride] : 0u;
+    const int sid = one_scene ? 0 : a.block_scene[g];
     const SceneDev sc = a.scenes[sid];
     const int local = (int)g - sc.block_base;
     const int nb = sc.nb;
@@ -1242,6 +1251,8 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.keys = d_sorted_keys;
   a.nchunks = d_nchunks;
   a.block_scene = d_block_scene;
+  a.nscenes = (int)scenes_host.size();
+  a.list_cap = (uint32_t)total_blocks;
   a.oob = d_oob;
   a.hard_list = d_hard;
   a.hard_count = d_counters + 3;

@@ -75,6 +75,8 @@ struct EngineArgs {
   const uint32_t* keys;  // binned (sorted) cell keys of the current order
   const uint32_t* nchunks;
   const int* block_scene;
+  int nscenes;        // scene table length (1: every block is scene 0's)
+  uint32_t list_cap;  // entries each active-block list has room for
   unsigned long long* oob;
   unsigned long long* fallback;
   double* sums;
