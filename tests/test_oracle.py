@@ -231,6 +231,28 @@ def test_grid_op_2d_brute_force():
     assert np.array_equal(gv, exp)
 
 
+@pytest.mark.parametrize("res,bound", [(16, 3), (20, 1), (12, 5)])
+def test_sticky_walls_node_counts(res, bound):
+    """Sticky walls of thickness `bound`: the untouched nodes are exactly the
+    (res - 2 bound + 1)^d interior lattice (counted, not re-derived per node),
+    and in 2D only nodes with mass are touched."""
+    n1 = res + 1
+    inner = res - 2 * bound + 1
+    gv = np.ones((n1, n1, n1, 3))
+    O.grid_op_3d(res, 1.0 / res, 1e-4, 0.0, gv, np.ones((n1, n1, n1, 1)), boundary="sticky", bound=bound)
+    zero = np.all(gv == 0.0, axis=-1)
+    assert zero.sum() == n1 ** 3 - inner ** 3
+    core = slice(bound, res - bound + 1)
+    assert np.all(gv[core, core, core] == 1.0) and not zero[core, core, core].any()
+    gv2 = np.full((n1, n1, 2), 2.0)
+    gm2 = np.full((n1, n1), 2.0)
+    gm2[0, :] = 0.0  # massless nodes on a wall keep their value
+    O.grid_op_2d(res, 1e-4, 0.0, gv2, gm2, boundary="sticky", bound=bound)
+    zero2 = np.all(gv2 == 0.0, axis=-1)
+    assert zero2.sum() == n1 ** 2 - inner ** 2 - n1
+    assert np.all(gv2[0, :] == 2.0) and np.all(gv2[core, core] == 1.0)
+
+
 def test_snow_gauges_agree_on_mechanics():
     st, pr = O.scene_3d_cubes(n=2000, res=32)
     st["kind"][:] = O.SNOW
