@@ -46,6 +46,12 @@ class MaterialSpec:
 class Engine:
     def __init__(self, scenes: Sequence[SceneSpec], materials: Sequence[MaterialSpec], n_max: int,
                  device=None):
+        bcs = {"reference": _lib.MPM_BC_REFERENCE, "sticky": _lib.MPM_BC_STICKY}
+        qks = {"reference": _lib.MPM_QUIRKS_REFERENCE, "textbook": _lib.MPM_QUIRKS_TEXTBOOK}
+        for i, s in enumerate(scenes):
+            if s.boundary not in bcs or s.quirks not in qks:
+                raise ValueError(f"scene {i}: boundary must be 'reference' or 'sticky' and quirks "
+                                 f"'reference' or 'textbook' (got {s.boundary!r}, {s.quirks!r})")
         _lib.require_gpu()
         self._L = _lib.load()
         self.device = torch.device(device) if device is not None else torch.device(
@@ -54,10 +60,7 @@ class Engine:
         self.materials = list(materials)
         sc = (_lib.Scene * len(self.scenes))()
         for i, s in enumerate(self.scenes):
-            sc[i] = _lib.Scene(int(s.res),
-                               _lib.MPM_BC_REFERENCE if s.boundary == "reference" else _lib.MPM_BC_STICKY,
-                               int(s.bound),
-                               _lib.MPM_QUIRKS_REFERENCE if s.quirks == "reference" else _lib.MPM_QUIRKS_TEXTBOOK,
+            sc[i] = _lib.Scene(int(s.res), bcs[s.boundary], int(s.bound), qks[s.quirks],
                                float(s.dt), float(s.gravity), float(s.volume))
         mt = (_lib.Material * len(self.materials))()
         for i, m in enumerate(self.materials):

@@ -29,6 +29,25 @@ def g2p_kind(model: str) -> int:
     return _lib.MPM_SNOW if model == "snow" else _lib.MPM_ELASTIC
 
 
+def boundary_code(boundary: str) -> int:
+    """Grid wall rule: "reference" (three_d/grid_op.py:37-47 slip walls,
+    two_d/grid_op.py:13-24 sticky sides and floor) or "sticky" (every
+    component zero within `bound` nodes of any wall)."""
+    codes = {"reference": _lib.MPM_BC_REFERENCE, "sticky": _lib.MPM_BC_STICKY}
+    if boundary not in codes:
+        raise ValueError(f"boundary must be 'reference' or 'sticky', got {boundary!r}")
+    return codes[boundary]
+
+
+def quirks_code(quirks: str) -> int:
+    """Stress rule: "reference" adds lambda (J-1) J to every entry of PF
+    (utils.py:129) or "textbook" adds it to the diagonal."""
+    codes = {"reference": _lib.MPM_QUIRKS_REFERENCE, "textbook": _lib.MPM_QUIRKS_TEXTBOOK}
+    if quirks not in codes:
+        raise ValueError(f"quirks must be 'reference' or 'textbook', got {quirks!r}")
+    return codes[quirks]
+
+
 def make_params(dim, res, inv_dx=0.0, dx=0.0, dt=1.0, volume=0.0, gravity=0.0,
                 boundary=_lib.MPM_BC_REFERENCE, bound=3, quirks=_lib.MPM_QUIRKS_REFERENCE,
                 check_oob=None):

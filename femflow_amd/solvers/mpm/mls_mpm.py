@@ -29,13 +29,16 @@ def make_mls_mpm_coefficients(lenx: int, dim: int, device=None, dtype=None):
 _ENGINES = {}
 
 
-def _engine_for(res, dt, gravity, volume, hardening, n, device):
-    key = (int(res), float(dt), float(gravity), float(volume), float(hardening), str(device))
+def _engine_for(res, dt, gravity, volume, hardening, n, device, boundary="reference", bound=3,
+                quirks="reference"):
+    key = (int(res), float(dt), float(gravity), float(volume), float(hardening), str(device), boundary,
+           int(bound), quirks)
     eng = _ENGINES.get(key)
     if eng is None or eng.n_max < n:
         if eng is not None:
             eng.close()
-        eng = Engine([SceneSpec(res=res, dt=dt, gravity=gravity, volume=volume)],
+        eng = Engine([SceneSpec(res=res, dt=dt, gravity=gravity, volume=volume, boundary=boundary,
+                                bound=int(bound), quirks=quirks)],
                      [MaterialSpec("elastic", hardening)], n_max=max(n, 1), device=device)
         eng.tune(sort_every=1)
         _ENGINES[key] = eng
@@ -43,18 +46,25 @@ def _engine_for(res, dt, gravity, volume, hardening, n, device):
 
 
 def solve_mls_mpm_3d(res: int, inv_dx: float, hardening: float, dx: float, dt: float, volume: float,
-                     gravity: float, particles, v, F, C, Jp):
+                     gravity: float, particles, v, F, C, Jp, *, boundary: str = "reference", bound: int = 3,
+                     quirks: str = "reference"):
     """mls_mpm.py:40-79: zero grids, P2G ("neo_hookean"), grid update, G2P.
 
     Mutates the particle positions and v, F, C, Jp in place.  FP64 inputs run
     the FP64 phase kernels on a dense (res+1)^3 grid; FP32 CUDA tensors run
-    the FP32 engine (binned, LDS-staged transfers) for one substep."""
+    the FP32 engine (binned, LDS-staged transfers) for one substep.  The
+    keyword-only options select the sticky walls (boundary="sticky", `bound`
+    nodes thick) and the textbook stress (quirks="textbook"); their defaults
+    are the reference's behaviour."""
+    bc = _phase.boundary_code(boundary)
+    q = _phase.quirks_code(quirks)
     sp = _phase.stage_3d(particles, v, F, C, Jp)
     if sp.dtype == torch.float64:
         dres = res + 1
         gv = torch.zeros((dres, dres, dres, 3), dtype=torch.float64, device=sp.device)
         gm = torch.zeros((dres, dres, dres, 1), dtype=torch.float64, device=sp.device)
-        prm = _phase.make_params(3, res, inv_dx, dx, dt, volume, gravity)
+        prm = _phase.make_params(3, res, inv_dx, dx, dt, volume, gravity, boundary=bc, bound=int(bound),
+                                 quirks=q)
         n_oob = _phase.run_p2g(sp, prm, _lib.MPM_ELASTIC, hardening, gv, gm)
         if n_oob:
             raise RuntimeError(f"{n_oob} particle(s) outside the grid")
@@ -66,7 +76,7 @@ def solve_mls_mpm_3d(res: int, inv_dx: float, hardening: float, dx: float, dt: f
         return
     if abs(inv_dx * dx - 1.0) > 1e-9 or abs(inv_dx - res) > 1e-9 * res:
         raise ValueError("the FP32 engine runs on [0,1]^3 with dx = 1/res")
-    eng = _engine_for(res, dt, gravity, volume, hardening, sp.n, sp.device)
+    eng = _engine_for(res, dt, gravity, volume, hardening, sp.n, sp.device, boundary, bound, quirks)
     jp = sp.Jp.tensor.view(-1)
     eng.upload(sp.x.tensor, sp.v.tensor, sp.F.tensor, sp.C.tensor, jp, sp.mass.tensor.contiguous(),
                sp.mu0.tensor.contiguous(), sp.lam0.tensor.contiguous())

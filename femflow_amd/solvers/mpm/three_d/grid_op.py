@@ -5,10 +5,15 @@ from __future__ import annotations
 from .. import _phase
 
 
-def grid_op(grid_resolution, dx, dt, gravity, grid_velocity, grid_mass):
+def grid_op(grid_resolution, dx, dt, gravity, grid_velocity, grid_mass, *, boundary: str = "reference",
+            bound: int = 3):
     """Momentum -> velocity, gravity on axis 1, clamp to 0.9 dx/dt, slip walls
-    one node thick at the low side and two at the high side (in place)."""
-    prm = _phase.make_params(3, grid_resolution, 1.0 / dx if dx else 0.0, dx, dt, 0.0, gravity)
+    one node thick at the low side and two at the high side (in place).
+    boundary="sticky" zeroes every component within `bound` nodes of a wall
+    instead of the slip walls."""
+    bc = _phase.boundary_code(boundary)
+    prm = _phase.make_params(3, grid_resolution, 1.0 / dx if dx else 0.0, dx, dt, 0.0, gravity,
+                             boundary=bc, bound=int(bound))
     _phase.run_grid_op(prm, grid_velocity, grid_mass)
 
 

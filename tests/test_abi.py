@@ -85,3 +85,23 @@ def test_package_never_imports_oracle():
                 with open(os.path.join(dirpath, fn)) as f:
                     src = f.read()
                 assert "import oracle" not in src and "from oracle" not in src, fn
+
+
+def test_option_strings_are_validated():
+    import numpy as np
+
+    from femflow_amd.engine import Engine, MaterialSpec, SceneSpec
+    from femflow_amd.solvers.mpm import _phase, three_d, two_d
+
+    assert _phase.boundary_code("sticky") == _lib.MPM_BC_STICKY
+    assert _phase.quirks_code("textbook") == _lib.MPM_QUIRKS_TEXTBOOK
+    g3, m3 = np.zeros((9, 9, 9, 3)), np.zeros((9, 9, 9, 1))
+    with pytest.raises(ValueError):
+        three_d.grid_op(8, 1 / 8, 1e-3, -9.8, g3, m3, boundary="slippery")
+    with pytest.raises(ValueError):
+        two_d.grid_op(8, 1e-3, -9.8, np.zeros((9, 9, 2)), np.zeros((9, 9)), boundary="Sticky")
+    with pytest.raises(ValueError):
+        _phase.quirks_code("numba")
+    with pytest.raises(ValueError):
+        Engine([SceneSpec(res=16, dt=1e-4, boundary="open")], [MaterialSpec("elastic", 1.0)], n_max=8,
+               device="cpu")
