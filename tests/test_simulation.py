@@ -8,25 +8,24 @@ from femflow_amd.simulation.mpm import primitives as P
 
 
 # ------------------------------------------------------------------ CPU ----
-def _grid_loop(res):
-    """The reference's loop (numerics/geometry.py:100-116), tiny sizes only."""
-    res = np.asarray(res)
-    vertices = np.empty((np.prod(res), len(res)))
-    sub = np.zeros(res.shape)
-    for i in range(vertices.shape[0]):
-        for c in range(len(res) - 1):
-            if sub[c] >= res[c]:
-                sub[c] = 0
-                sub[c + 1] += 1
-        for c in range(len(res)):
-            vertices[i, c] = sub[c] / res[c] if res[c] == 1 else sub[c] / (res[c] - 1)
-        sub[0] += 1
-    return vertices
+def test_grid_reference_kat():
+    """numerics/tests/test_geometry.py:6-40: res (3, 3, 3) gives the 27 points
+    of {0, 0.5, 1}^3 with the first coordinate varying fastest."""
+    h = (0.0, 0.5, 1.0)
+    exp = np.array([[x, y, z] for z in h for y in h for x in h])
+    assert np.array_equal(P.grid(np.array((3, 3, 3))), exp)
 
 
 @pytest.mark.parametrize("res", [(3, 2, 4), (1, 3, 2), (5, 5, 5)])
-def test_grid_matches_reference_loop(res):
-    assert np.array_equal(P.grid(np.array(res)), _grid_loop(res))
+def test_grid_mixed_radix(res):
+    """Row i holds the mixed-radix digits of i (first axis least significant),
+    each divided by res_c - 1, or by res_c on an axis of one point."""
+    g = P.grid(np.array(res))
+    i = np.arange(int(np.prod(res)))
+    digits = (i % res[0], (i // res[0]) % res[1], i // (res[0] * res[1]))
+    assert g.shape == (len(i), 3)
+    for c in range(3):
+        assert np.array_equal(g[:, c], digits[c] / (res[c] if res[c] == 1 else res[c] - 1))
 
 
 def test_cube_points_column_order():
