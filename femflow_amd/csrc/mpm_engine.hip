@@ -831,12 +831,21 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
         q.z += sv[c][sn].z;
         q.w += sv[c][sn].w;
       }
-      for (uint32_t c = 3; c < cn[sn]; ++c) {
-        const float4 v2 = a.stage[(size_t)(c0[sn] + c) * STAGE_NODES + ai[sn]];
-        q.x += v2.x;
-        q.y += v2.y;
-        q.z += v2.z;
-        q.w += v2.w;
+      // dense sources (many chunks per block): eight loads in flight per
+      // round, summed in the same chunk order
+      for (uint32_t c = 3; c < cn[sn]; c += 8) {
+        float4 v8[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          v8[u] = c + u < cn[sn] ? a.stage[(size_t)(c0[sn] + c + u) * STAGE_NODES + ai[sn]]
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          q.x += v8[u].x;
+          q.y += v8[u].y;
+          q.z += v8[u].z;
+          q.w += v8[u].w;
+        }
       }
     }
     if (a.pm) a.pm[addr] = q;
