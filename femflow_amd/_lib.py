@@ -25,8 +25,9 @@ EXPORTS = (
     "mpm_last_error_string", "mpm_version", "mpm_device_count", "mpm_p2g", "mpm_grid_op",
     "mpm_g2p", "mpm_ctx_create", "mpm_ctx_destroy", "mpm_ctx_set_stream", "mpm_ctx_tune",
     "mpm_ctx_kernel_timing", "mpm_ctx_kernel_times", "mpm_state_upload", "mpm_state_download",
-    "mpm_run", "mpm_run_frames", "mpm_run_states", "mpm_state_positions", "mpm_substep", "mpm_ctx_oob_count", "mpm_ctx_fallback_count",
+    "mpm_run", "mpm_run_frames", "mpm_run_states", "mpm_run_states_grids", "mpm_state_positions", "mpm_substep", "mpm_ctx_oob_count", "mpm_ctx_fallback_count",
     "mpm_particle_sums", "mpm_bin", "mpm_dense_workspace_bytes", "mpm_dense_substep", "mpm_adjoint_step",
+    "mpm_adjoint_step_grids",
     "mpm_ctx_set_corrector", "mpm_corrector_sample", "mpm_grid_velocity",
     "mpm_collide_planes", "mpm_ctx_set_planes",
     "mpm_mlp_sizes", "mpm_mlp_pack", "mpm_mlp_forward", "mpm_mlp_loss", "mpm_mlp_backward", "mpm_mlp_adam",
@@ -101,6 +102,7 @@ def load():
             "mpm_run": ([vp, i32, vp], i32),
             "mpm_run_frames": ([vp, i32, vp], i32),
             "mpm_run_states": ([vp, i32, vp, vp, vp, vp, vp], i32),
+            "mpm_run_states_grids": ([vp, i32, vp, vp, vp, vp, vp, vp], i32),
             "mpm_state_positions": ([vp, vp], i32),
             "mpm_substep": ([vp, P, vp, i32], i32),
             "mpm_ctx_oob_count": ([vp, ct.POINTER(i64)], i32),
@@ -110,6 +112,7 @@ def load():
             "mpm_dense_workspace_bytes": ([Q, ct.POINTER(i64)], i32),
             "mpm_dense_substep": ([vp, Q, P, dbl, vp, vp], i32),
             "mpm_adjoint_step": ([vp, Q, P, dbl, vp, vp, vp, vp, vp, vp, vp, vp], i32),
+            "mpm_adjoint_step_grids": ([vp, Q, P, dbl, vp, vp, vp, vp, vp, vp, vp, vp, vp], i32),
             "mpm_ctx_set_corrector": ([vp, vp, ct.c_float, ct.c_float, i32], i32),
             "mpm_corrector_sample": ([vp, vp, vp, vp, i64, ct.POINTER(i64)], i32),
             "mpm_grid_velocity": ([vp, vp, i64, vp], i32),

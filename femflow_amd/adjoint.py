@@ -9,7 +9,8 @@ reverse order.  The forward substeps run on the particle-resident engine and
 keep a checkpoint of the particle state every ``cfg.checkpoint_every``
 substeps; the backward replays one checkpoint interval at a time on the
 engine, keeping that interval's per-substep states (24 floats per particle
-per substep), and reverses it.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
+per substep) and dense forward grids (7 floats per node per substep, from
+``mpm_run_states_grids``), and reverses it with ``mpm_adjoint_step_grids``.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
 or scalar); mass is a constant.
 
 The reference has no gradients; the forward it differentiates is its step
@@ -92,6 +93,16 @@ class DenseStepper:
                                            vb.data_ptr(), Fb.data_ptr(), Cb.data_ptr(), mub.data_ptr(),
                                            lamb.data_ptr(), self.oob.data_ptr()), "mpm_adjoint_step")
 
+    def adjoint_grids(self, x, v, F, C, mass, mu0, lam0, grids, xb, vb, Fb, Cb, mub, lamb):
+        """adjoint() with the substep's forward grids given (one slot of
+        EngineStepper.replay) instead of recomputed from the state."""
+        p = self._particles(x, v, F, C, mass, mu0, lam0)
+        _lib.check(self.L.mpm_adjoint_step_grids(stream_ptr(self.device), ct.byref(self.prm), ct.byref(p),
+                                                 float(self.cfg.hardening), grids.data_ptr(), self.work.data_ptr(),
+                                                 xb.data_ptr(), vb.data_ptr(), Fb.data_ptr(), Cb.data_ptr(),
+                                                 mub.data_ptr(), lamb.data_ptr(), self.oob.data_ptr()),
+                   "mpm_adjoint_step_grids")
+
     def check_oob(self):
         n = int(self.oob.item())
         if n:
@@ -113,6 +124,7 @@ class EngineStepper:
                                      boundary=cfg.boundary, bound=cfg.bound, quirks=cfg.quirks)],
                           [MaterialSpec("elastic", cfg.hardening)], n_max=max(self.n, 1), device=self.device)
         self.eng.tune(sort_every=16)
+        self.res = int(cfg.res)
         full = lambda t: (t.expand(self.n) if t.numel() == 1 else t.reshape(self.n)).contiguous()  # noqa: E731
         self.mass, self.mu0, self.lam0 = full(mass), full(mu0), full(lam0)
         self.Jp = torch.ones(self.n, dtype=torch.float32, device=self.device)
@@ -126,12 +138,17 @@ class EngineStepper:
         self.eng.run(nsteps, check_oob=False)
 
     def replay(self, nsteps: int):
-        """States after each of the next ``nsteps`` substeps (one engine run)."""
+        """The next ``nsteps`` substeps in one engine run: the states after
+        each substep and each substep's dense forward grids (P2G momentum,
+        mass, updated velocity), which the reverse pass reads instead of
+        recomputing them."""
         n, d = self.n, self.device
         x, v = torch.empty((nsteps, n, 3), device=d), torch.empty((nsteps, n, 3), device=d)
         F, C = torch.empty((nsteps, n, 3, 3), device=d), torch.empty((nsteps, n, 3, 3), device=d)
-        self.eng.run_states(nsteps, x, v, F, C, torch.empty((nsteps, n), device=d), check_oob=False)
-        return [(x[k], v[k], F[k], C[k]) for k in range(nsteps)]
+        r1 = self.res + 1
+        grids = torch.empty((nsteps, 7 * r1 ** 3), device=d)
+        self.eng.run_states(nsteps, x, v, F, C, torch.empty((nsteps, n), device=d), check_oob=False, grids=grids)
+        return [(x[k], v[k], F[k], C[k]) for k in range(nsteps)], grids
 
     def state(self):
         n, d = self.n, self.device
@@ -189,12 +206,15 @@ class _Rollout(torch.autograd.Function):
         for seg in range(len(ctx.ckpts) - 1, -1, -1):
             s0 = seg * K
             s1 = min(s0 + K, cfg.nsteps)
-            # recompute the interval's per-substep input states on the engine
+            # replay the interval on the engine: each substep's input state and
+            # its forward grids
             fwd.load(*ctx.ckpts[seg])
-            states = [ctx.ckpts[seg]] + (fwd.replay(s1 - s0 - 1) if s1 - s0 > 1 else [])
-            for (x, v, F, C) in reversed(states):
-                st.adjoint(x, v, F, C, mass_, mu_, lam_, xb, vb, Fb, Cb, mub, lamb)
-            del states
+            after, grids = fwd.replay(s1 - s0)
+            states = [ctx.ckpts[seg]] + after[:-1]
+            for k in range(s1 - s0 - 1, -1, -1):
+                x, v, F, C = states[k]
+                st.adjoint_grids(x, v, F, C, mass_, mu_, lam_, grids[k], xb, vb, Fb, Cb, mub, lamb)
+            del states, after, grids
         fwd.check_oob()
         st.check_oob()
         fwd.close()

@@ -418,11 +418,12 @@ void dense_forward_grids(hipStream_t st, const mpm_params* prm, const mpm_partic
   phase_grid_op(st, prm, MPM_F32, gv, gm);
 }
 
-void adjoint_step(hipStream_t st, const mpm_params* prm, const mpm_particles* p, double hardening,
-                  float* work, float* xb, float* vb, float* Fb, float* Cb, float* mub, float* lamb,
-                  unsigned long long* oob) {
+// Reverse substep from the forward grids `grids` (gp, gm, gv as laid out at
+// the start of the workspace); the gradient grid lives at work + 7 nn.
+static void adjoint_from_grids(hipStream_t st, const mpm_params* prm, const mpm_particles* p, double hardening,
+                               const float* grids, float* work, float* xb, float* vb, float* Fb, float* Cb,
+                               float* mub, float* lamb, unsigned long long* oob) {
   const int64_t r1 = prm->res + 1, nn = r1 * r1 * r1;
-  dense_forward_grids(st, prm, p, hardening, work, oob);
   AdjArgs a{};
   a.n = p->n;
   a.res = prm->res;
@@ -444,9 +445,9 @@ void adjoint_step(hipStream_t st, const mpm_params* prm, const mpm_particles* p,
   a.mass = (const float*)p->mass;
   a.mu0 = (const float*)p->mu0;
   a.lam0 = (const float*)p->lambda0;
-  a.gp = work;
-  a.gm = work + 3 * nn;
-  a.gv = work + 4 * nn;
+  a.gp = grids;
+  a.gm = grids + 3 * nn;
+  a.gv = grids + 4 * nn;
   a.gbar = work + 7 * nn;
   a.xb = xb;
   a.vb = vb;
@@ -459,6 +460,19 @@ void adjoint_step(hipStream_t st, const mpm_params* prm, const mpm_particles* p,
   if (a.n > 0) hipLaunchKernelGGL(adj_g2p_kernel, dim3(nblk(a.n, 128)), dim3(128), 0, st, a);
   hipLaunchKernelGGL(adj_grid_kernel, dim3(nblk(nn, 256)), dim3(256), 0, st, a);
   if (a.n > 0) hipLaunchKernelGGL(adj_p2g_kernel, dim3(nblk(a.n, 128)), dim3(128), 0, st, a);
+}
+
+void adjoint_step(hipStream_t st, const mpm_params* prm, const mpm_particles* p, double hardening,
+                  float* work, float* xb, float* vb, float* Fb, float* Cb, float* mub, float* lamb,
+                  unsigned long long* oob) {
+  dense_forward_grids(st, prm, p, hardening, work, oob);
+  adjoint_from_grids(st, prm, p, hardening, work, work, xb, vb, Fb, Cb, mub, lamb, oob);
+}
+
+void adjoint_step_grids(hipStream_t st, const mpm_params* prm, const mpm_particles* p, double hardening,
+                        const float* grids, float* work, float* xb, float* vb, float* Fb, float* Cb, float* mub,
+                        float* lamb, unsigned long long* oob) {
+  adjoint_from_grids(st, prm, p, hardening, grids, work, xb, vb, Fb, Cb, mub, lamb, oob);
 }
 
 }  // namespace mpm

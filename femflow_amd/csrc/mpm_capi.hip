@@ -134,6 +134,21 @@ int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particle
   return MPM_OK;
 }
 
+int32_t mpm_adjoint_step_grids(void* stream, const mpm_params* prm, const mpm_particles* p, double hardening,
+                               const float* grids, void* work, float* x_bar, float* v_bar, float* F_bar,
+                               float* C_bar, float* mu0_bar, float* lambda0_bar, int64_t* oob_count) {
+  int rc;
+  if ((rc = check_dense(prm, p, work, "mpm_adjoint_step_grids"))) return rc;
+  if (!grids || (p->n > 0 && (!x_bar || !v_bar || !F_bar || !C_bar || !mu0_bar || !lambda0_bar))) {
+    set_error("mpm_adjoint_step_grids: null grid or adjoint array");
+    return MPM_ERR_ARG;
+  }
+  mpm::adjoint_step_grids((hipStream_t)stream, prm, p, hardening, grids, (float*)work, x_bar, v_bar, F_bar,
+                          C_bar, mu0_bar, lambda0_bar, (unsigned long long*)oob_count);
+  CHECK_HIP(hipGetLastError());
+  return MPM_OK;
+}
+
 int32_t mpm_ctx_set_corrector(mpm_ctx* ctx, const float* params, float m_ref, float v_ref,
                               int32_t enable) {
   if (!ctx) {
@@ -409,7 +424,21 @@ int32_t mpm_run_states(mpm_ctx* ctx, int32_t nsteps, float* x, float* v, float* 
     set_error("mpm_run_states: null argument");
     return MPM_ERR_ARG;
   }
-  const mpm::StateSink sink{x, v, F, C, Jp};
+  const mpm::StateSink sink{x, v, F, C, Jp, nullptr};
+  return ctx->eng.run(nsteps, nullptr, nullptr, &sink);
+}
+
+int32_t mpm_run_states_grids(mpm_ctx* ctx, int32_t nsteps, float* x, float* v, float* F, float* C, float* Jp,
+                             float* grids) {
+  if (!ctx || (nsteps > 0 && ctx->eng.n > 0 && (!x || !v || !F || !C || !Jp)) || (nsteps > 0 && !grids)) {
+    set_error("mpm_run_states_grids: null argument");
+    return MPM_ERR_ARG;
+  }
+  if (ctx->eng.scenes_host.size() != 1 || ctx->eng.corr_on) {
+    set_error("mpm_run_states_grids: needs a single-scene context without a corrector");
+    return MPM_ERR_ARG;
+  }
+  const mpm::StateSink sink{x, v, F, C, Jp, grids};
   return ctx->eng.run(nsteps, nullptr, nullptr, &sink);
 }
 

@@ -31,9 +31,13 @@ struct CorrExtract {
 };
 
 // Per-substep state output (caller order): slot k of each array receives the
-// state after substep k.
+// state after substep k.  grids (optional, single-scene engines): slot k of
+// 7 (res+1)^3 floats receives substep k's dense grids, the P2G momentum
+// ((res+1)^3 x 3), the P2G mass ((res+1)^3) and the updated velocity
+// ((res+1)^3 x 3), zero at nodes no particle reached.
 struct StateSink {
   float *x, *v, *F, *C, *Jp;
+  float* grids;
 };
 
 struct TimedLaunch {

@@ -881,6 +881,19 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       if (a.planes[pl].scene == sid && plane_blocks(a.planes[pl], i, j, k)) v[0] = v[1] = v[2] = 0.f;
     a.vel[addr] = make_float4(v[0], v[1], v[2], m);
     a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (a.grid_rec && i <= sc.res && j <= sc.res && k <= sc.res) {
+      const size_t r1 = (size_t)sc.res + 1, nn = r1 * r1 * r1;
+      const size_t di = ((size_t)i * r1 + (size_t)j) * r1 + (size_t)k;
+      float* gp = a.grid_rec;
+      gp[3 * di] = q.x;
+      gp[3 * di + 1] = q.y;
+      gp[3 * di + 2] = q.z;
+      gp[3 * nn + di] = m;
+      float* gv = gp + 4 * nn;
+      gv[3 * di] = v[0];
+      gv[3 * di + 1] = v[1];
+      gv[3 * di + 2] = v[2];
+    }
     if (with_sums) {
       s[4] += (double)m * v[0];
       s[5] += (double)m * v[1];
@@ -1425,16 +1438,22 @@ void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps
   hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws);
   enqueue_bin(s);
   // substep k appends to active list k % 2; its grid update clears the other
+  const size_t rec_nn = (size_t)(scenes_host[0].res + 1) * (scenes_host[0].res + 1) * (scenes_host[0].res + 1);
+  auto grid_args = [&](int k) {
+    EngineArgs a = args(sums, max_steps, k & 1);
+    if (sink && sink->grids) a.grid_rec = sink->grids + (size_t)(step0 + k) * 7 * rec_nn;
+    return a;
+  };
   launch_transfer(0, s, args(sums, max_steps, 0));
   if (corr_on) launch_cellf(s, args(sums, max_steps, 0));
-  launch_grid(s, args(sums, max_steps, 0), ws, len, 0);
+  launch_grid(s, grid_args(0), ws, len, 0);
   if (corr_on) launch_corrector(s, args(sums, max_steps, 0), nullptr);
   for (int k = 1; k < len; ++k) {
     launch_transfer(1, s, args(sums, max_steps, k & 1));
     enqueue_frame(s, frames, step0 + k - 1);  // substep k-1 finished its G2P
     enqueue_state(s, sink, step0 + k - 1);
     if (corr_on) launch_cellf(s, args(sums, max_steps, k & 1));
-    launch_grid(s, args(sums, max_steps, k & 1), ws, len, k);
+    launch_grid(s, grid_args(k), ws, len, k);
     if (corr_on) launch_corrector(s, args(sums, max_steps, k & 1), nullptr);
   }
   launch_transfer(2, s, args(sums, max_steps, len & 1));
@@ -1454,6 +1473,12 @@ int Engine::run(int nsteps, double* sums, float* frames, const StateSink* sink) 
     hipLaunchKernelGGL(fill_f64_kernel, dim3(nblk((int64_t)nsteps * 8, 256)), dim3(256), 0, own_stream,
                        sums, (int64_t)nsteps * 8, 0.0);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, own_stream, d_counters + 2, (int64_t)1, 0u);
+  }
+  if (sink && sink->grids) {  // nodes no particle reaches stay zero
+    const int64_t r1 = scenes_host[0].res + 1;
+    const int64_t cnt = (int64_t)nsteps * 7 * r1 * r1 * r1;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(cnt, 256)), dim3(256), 0, own_stream,
+                       reinterpret_cast<uint32_t*>(sink->grids), cnt, 0u);
   }
   int done = 0;
   while (done < nsteps) {

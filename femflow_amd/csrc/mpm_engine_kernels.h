@@ -100,6 +100,9 @@ struct EngineArgs {
   // Sticky half-space colliders (null / 0 when none): see PlaneDev.
   const PlaneDev* planes;
   int nplanes;
+  // Dense grid record of this substep (null: none): scene 0's P2G momentum
+  // (nn x 3), P2G mass (nn) and updated velocity (nn x 3), nn = (res+1)^3.
+  float* grid_rec;
 };
 
 __device__ __forceinline__ size_t node_addr(const SceneDev& sc, int i, int j, int k) {

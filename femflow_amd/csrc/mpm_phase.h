@@ -25,5 +25,8 @@ void dense_forward_grids(hipStream_t st, const mpm_params* prm, const mpm_partic
 void adjoint_step(hipStream_t st, const mpm_params* prm, const mpm_particles* p, double hardening,
                   float* work, float* xb, float* vb, float* Fb, float* Cb, float* mub, float* lamb,
                   unsigned long long* oob);
+void adjoint_step_grids(hipStream_t st, const mpm_params* prm, const mpm_particles* p, double hardening,
+                        const float* grids, float* work, float* xb, float* vb, float* Fb, float* Cb, float* mub,
+                        float* lamb, unsigned long long* oob);
 
 }  // namespace mpm

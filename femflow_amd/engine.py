@@ -146,10 +146,12 @@ class Engine:
             if n:
                 raise RuntimeError(f"{n} particle stencil(s) left the grid")
 
-    def run_states(self, nsteps: int, x, v, F, C, Jp, check_oob: bool = True):
+    def run_states(self, nsteps: int, x, v, F, C, Jp, check_oob: bool = True, grids=None):
         """Advance ``nsteps`` substeps, writing the full state after substep k
         (caller order) to slot k: x, v (nsteps, N, 3), F, C (nsteps, N, 3, 3),
-        Jp (nsteps, N); float32 CUDA tensors."""
+        Jp (nsteps, N); float32 CUDA tensors.  grids (single-scene engines):
+        float32 (nsteps, 7 (res+1)^3) receiving each substep's dense P2G
+        momentum, P2G mass and updated velocity."""
         n = self.n
         for t, nm, shape in ((x, "x", (nsteps, n, 3)), (v, "v", (nsteps, n, 3)), (F, "F", (nsteps, n, 3, 3)),
                              (C, "C", (nsteps, n, 3, 3)), (Jp, "Jp", (nsteps, n))):
@@ -157,8 +159,17 @@ class Engine:
             if tuple(t.shape) != shape:
                 raise ValueError(f"{nm} must have shape {shape}")
         self._sync_stream()
-        _lib.check(self._L.mpm_run_states(self._h, int(nsteps), x.data_ptr(), v.data_ptr(), F.data_ptr(),
-                                          C.data_ptr(), Jp.data_ptr()), "run_states")
+        if grids is None:
+            _lib.check(self._L.mpm_run_states(self._h, int(nsteps), x.data_ptr(), v.data_ptr(), F.data_ptr(),
+                                              C.data_ptr(), Jp.data_ptr()), "run_states")
+        else:
+            self._check_tensor(grids, "grids")
+            r1 = self.scenes[0].res + 1
+            if grids.numel() != nsteps * 7 * r1 ** 3:
+                raise ValueError(f"grids must hold {nsteps} x 7 x {r1}^3 floats")
+            _lib.check(self._L.mpm_run_states_grids(self._h, int(nsteps), x.data_ptr(), v.data_ptr(),
+                                                    F.data_ptr(), C.data_ptr(), Jp.data_ptr(), grids.data_ptr()),
+                       "run_states")
         if check_oob:
             k = self.oob_count()
             if k:

@@ -179,6 +179,15 @@ int32_t mpm_run_frames(mpm_ctx* ctx, int32_t nsteps, float* frames);
  * (nsteps x n), FP32 device arrays.  No reference counterpart: it serves
  * differentiable rollouts, which replay the states of a checkpoint interval. */
 int32_t mpm_run_states(mpm_ctx* ctx, int32_t nsteps, float* x, float* v, float* F, float* C, float* Jp);
+/* mpm_run_states that also records each substep's dense grids: slot k of
+ * grids (device float, nsteps x 7 (res+1)^3) receives substep k's P2G
+ * momentum ((res+1)^3 x 3), P2G mass ((res+1)^3) and updated velocity
+ * ((res+1)^3 x 3), zero where no particle reached, in the layout
+ * mpm_adjoint_step_grids reads.  Single-scene contexts without a corrector.
+ * No reference counterpart: the reverse pass of a differentiable rollout
+ * takes its forward grids from here instead of recomputing them. */
+int32_t mpm_run_states_grids(mpm_ctx* ctx, int32_t nsteps, float* x, float* v, float* F, float* C, float* Jp,
+                             float* grids);
 /* Current positions in the caller's order (device float, n * 3). */
 int32_t mpm_state_positions(mpm_ctx* ctx, float* x);
 /* Convenience for the reference's per-call pattern: upload, run, download. */
@@ -223,6 +232,12 @@ int32_t mpm_dense_substep(void* stream, const mpm_params* prm, mpm_particles* p,
 int32_t mpm_adjoint_step(void* stream, const mpm_params* prm, const mpm_particles* p, double hardening,
                          void* work, float* x_bar, float* v_bar, float* F_bar, float* C_bar,
                          float* mu0_bar, float* lambda0_bar, int64_t* oob_count);
+/* mpm_adjoint_step with the substep's forward grids given (one slot of
+ * mpm_run_states_grids: P2G momentum, P2G mass, updated velocity) instead of
+ * recomputed from p; work is the same workspace (its gradient grid is used). */
+int32_t mpm_adjoint_step_grids(void* stream, const mpm_params* prm, const mpm_particles* p, double hardening,
+                               const float* grids, void* work, float* x_bar, float* v_bar, float* F_bar,
+                               float* C_bar, float* mu0_bar, float* lambda0_bar, int64_t* oob_count);
 
 /* ---- learned corrector inside an engine context --------------------------
  * No reference counterpart.  params: device FP32 MLP parameters (layout of

@@ -119,6 +119,60 @@ def test_adjoint_step_matches_fp64_autograd(gpu, quirks):
 
 
 @pytest.mark.gpu
+def test_recorded_engine_grids_match_dense_grids(gpu):
+    """The engine's per-substep grid record (P2G momentum, mass, updated
+    velocity) equals the dense grids the adjoint would rebuild from the same
+    state, and the adjoint step taken from the record matches the one that
+    rebuilds them."""
+    from femflow_amd.adjoint import DenseStepper, EngineStepper, RolloutConfig
+
+    n, res, dt = 4096, 32, 1e-4
+    sc = _cube_scene(n, res, 31, vscale=0.5)
+    cfg = RolloutConfig(res=res, dt=dt, volume=sc["vol"], nsteps=3, hardening=1.1)
+    f32 = lambda a: torch.tensor(np.asarray(a), dtype=torch.float32, device=gpu).contiguous()  # noqa: E731
+    x, v, F, C = f32(sc["x"]), f32(sc["v"]), f32(sc["F"]), f32(sc["C"])
+    mass, mu0, lam0 = f32(sc["mass"]), f32(sc["mu0"]), f32(sc["lam0"])
+    fwd = EngineStepper(cfg, n, gpu, mass, mu0, lam0)
+    fwd.load(x, v, F, C)
+    after, grids = fwd.replay(3)
+    states = [(x, v, F, C)] + after[:-1]
+    st = DenseStepper(cfg, n, gpu)
+    nn = (res + 1) ** 3
+    rng = np.random.default_rng(6)
+    for k in range(3):
+        xs, vs, Fs, Cs = (t.clone() for t in states[k])
+        st.step(xs, vs, Fs, Cs, mass, mu0, lam0)
+        torch.cuda.synchronize()
+        dense = st.work[:7 * nn].cpu().numpy()
+        rec = grids[k].cpu().numpy()
+        # velocity weighted by node mass: v = p / m amplifies FP32 rounding of
+        # the sums at nodes holding a sliver of mass, which the gather weighs
+        # by that same sliver
+        wm = np.repeat(dense[3 * nn:4 * nn], 3)
+        for name, a, b, w in (("momentum", 0, 3 * nn, 1.0), ("mass", 3 * nn, 4 * nn, 1.0),
+                              ("mass-weighted velocity", 4 * nn, 7 * nn, wm)):
+            r, d = rec[a:b] * w, dense[a:b] * w
+            e = np.abs(r - d).max() / np.abs(d).max()
+            print(f"substep {k} recorded {name}: rel err {e:.2e}", flush=True)
+            assert e < 1e-5, (k, name)
+        cot = [f32(rng.standard_normal(sh)) for sh in ((n, 3), (n, 3), (n, 3, 3), (n, 3, 3))]
+        outs = []
+        for use_rec in (False, True):
+            xb, vb, Fb, Cb = (c.clone() for c in cot)
+            mub, lamb = torch.zeros_like(mu0), torch.zeros_like(lam0)
+            if use_rec:
+                st.adjoint_grids(*states[k], mass, mu0, lam0, grids[k], xb, vb, Fb, Cb, mub, lamb)
+            else:
+                st.adjoint(*states[k], mass, mu0, lam0, xb, vb, Fb, Cb, mub, lamb)
+            outs.append([t.cpu().numpy() for t in (xb, vb, Fb, Cb, mub, lamb)])
+        for name, a, b in zip(("x", "v", "F", "C", "mu0", "lam0"), outs[0], outs[1]):
+            e = _rel(b, a)
+            assert e < 1e-4, (k, name, e)
+    st.check_oob()
+    fwd.close()
+
+
+@pytest.mark.gpu
 def test_rollout_gradient_matches_fp64_reference(gpu):
     from femflow_amd.adjoint import RolloutConfig, rollout
 
