@@ -18,6 +18,8 @@
 // fixed and a clamped or masked component passes no gradient.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "mpm355.h"
 #include "mpm_math.h"
 #include "mpm_phase.h"
@@ -70,7 +72,32 @@ __device__ __forceinline__ void bspline_d(float fx, float w[3], float dw[3]) {
 }
 
 // ---------------------------------------------------------------- G2P^T ----
-__global__ __launch_bounds__(128) void adj_g2p_kernel(AdjArgs a) {
+// The node-velocity adjoint is scattered in two levels: a wave sums the lanes
+// of each run of particles in one cell (seg_reduce), and the workgroup sums
+// those into an LDS arena over the bounding box of its particles' stencils,
+// which it then adds to the grid with one device atomic per nonzero node
+// component.  Particles in cell order give a box of a few cells, so most of
+// the adds stay on the chip; device-scope atomics, which serialise across
+// the XCDs, were the kernel's cost.  A box larger than the arena falls back
+// to one device atomic per segment.
+constexpr int AG_THREADS = 256;
+constexpr int AG_CAP = 1024;  // arena nodes (x 3 components)
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+__global__ __launch_bounds__(AG_THREADS) void adj_g2p_kernel(AdjArgs a) {
+  __shared__ float arena[AG_CAP * 3];
+  __shared__ int box_w[AG_THREADS / 64][6];
+  __shared__ int box[6];  // low corner (3), extent (3) of the stencil box
   // lanes past the end or out of the grid stay in the wave (its segmented
   // atomics need every lane) and write nothing
   const int64_t p0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,6 +177,46 @@ __global__ __launch_bounds__(128) void adj_g2p_kernel(AdjArgs a) {
 #pragma unroll
   for (int c = 0; c < 3; ++c) gx[c] = -c4 * (Ct[c] * vn[0] + Ct[3 + c] * vn[1] + Ct[6 + c] * vn[2]);
   const WaveSeg sg = wave_segments(valid, (int)((base[0] * r1 + base[1]) * r1 + base[2]));
+  {
+    const int t = threadIdx.x, wv = t >> 6;
+    int lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = wave_min(valid ? base[d] : INT_MAX);
+      hi[d] = wave_max(valid ? base[d] : INT_MIN);
+    }
+    if ((t & 63) == 0) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        box_w[wv][d] = lo[d];
+        box_w[wv][3 + d] = hi[d];
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      int l[3] = {INT_MAX, INT_MAX, INT_MAX}, h[3] = {INT_MIN, INT_MIN, INT_MIN};
+      for (int w2 = 0; w2 < AG_THREADS / 64; ++w2)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          l[d] = min(l[d], box_w[w2][d]);
+          h[d] = max(h[d], box_w[w2][3 + d]);
+        }
+      const bool any = l[0] <= h[0];
+      const int64_t e0 = any ? h[0] - l[0] + 3 : 0, e1 = any ? h[1] - l[1] + 3 : 0, e2 = any ? h[2] - l[2] + 3 : 0;
+      const bool fits = any && e0 * e1 * e2 <= AG_CAP;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) box[d] = l[d];
+      box[3] = fits ? (int)e0 : 0;
+      box[4] = fits ? (int)e1 : 0;
+      box[5] = fits ? (int)e2 : 0;
+    }
+    __syncthreads();
+    const int vol = box[3] * box[4] * box[5];
+    for (int e = t; e < 3 * vol; e += AG_THREADS) arena[e] = 0.f;
+    __syncthreads();
+  }
+  const int bl0 = box[0], bl1 = box[1], bl2 = box[2], be1 = box[4], be2 = box[5];
+  const bool use_arena = box[3] > 0;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j)
 #pragma unroll
@@ -164,10 +231,14 @@ __global__ __launch_bounds__(128) void adj_g2p_kernel(AdjArgs a) {
         for (int d = 0; d < 3; ++d)
           gb[d] = vt[d] + c4 * (Ct[3 * d] * dp[0] + Ct[3 * d + 1] * dp[1] + Ct[3 * d + 2] * dp[2]);
         float* dst = a.gbar + 4 * node;
+        const int slot = ((base[0] + i - bl0) * be1 + (base[1] + j - bl1)) * be2 + (base[2] + k - bl2);
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           const float r = seg_reduce(wt * gb[d], sg);
-          if (sg.tail) atomicAdd(dst + d, r);
+          if (sg.tail) {
+            if (use_arena) atomicAdd(&arena[3 * slot + d], r);
+            else atomicAdd(dst + d, r);
+          }
         }
         // weight derivative
         const float s = g[0] * gb[0] + g[1] * gb[1] + g[2] * gb[2];
@@ -175,11 +246,23 @@ __global__ __launch_bounds__(128) void adj_g2p_kernel(AdjArgs a) {
         gx[1] += w[0][i] * dw[1][j] * w[2][k] * s;
         gx[2] += w[0][i] * w[1][j] * dw[2][k] * s;
       }
-  if (!valid) return;
+  if (valid) {
 #pragma unroll
-  for (int d = 0; d < 3; ++d) a.xb[3 * p + d] = xo[d] + a.inv_dx * gx[d];
+    for (int d = 0; d < 3; ++d) a.xb[3 * p + d] = xo[d] + a.inv_dx * gx[d];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) a.Fb[9 * p + e] = Fbar[e];
+    for (int e = 0; e < 9; ++e) a.Fb[9 * p + e] = Fbar[e];
+  }
+  if (!use_arena) return;  // workgroup-uniform
+  __syncthreads();
+  const int e1 = box[4], e2 = box[5], vol = box[3] * e1 * e2;
+  for (int e = threadIdx.x; e < 3 * vol; e += AG_THREADS) {
+    const float val = arena[e];
+    if (val != 0.f) {
+      const int sl = e / 3, d = e - 3 * sl;
+      const int64_t I = bl0 + sl / (e1 * e2), J = bl1 + (sl / e2) % e1, K = bl2 + sl % e2;
+      atomicAdd(a.gbar + 4 * ((I * r1 + J) * r1 + K) + d, val);
+    }
+  }
 }
 
 // --------------------------------------------------------------- grid^T ----
@@ -457,7 +540,7 @@ static void adjoint_from_grids(hipStream_t st, const mpm_params* prm, const mpm_
   a.lamb = lamb;
   a.oob = oob;
   hipLaunchKernelGGL(adj_fill_kernel, dim3(nblk(4 * nn, 256)), dim3(256), 0, st, a.gbar, 4 * nn, 0.f);
-  if (a.n > 0) hipLaunchKernelGGL(adj_g2p_kernel, dim3(nblk(a.n, 128)), dim3(128), 0, st, a);
+  if (a.n > 0) hipLaunchKernelGGL(adj_g2p_kernel, dim3(nblk(a.n, AG_THREADS)), dim3(AG_THREADS), 0, st, a);
   hipLaunchKernelGGL(adj_grid_kernel, dim3(nblk(nn, 256)), dim3(256), 0, st, a);
   if (a.n > 0) hipLaunchKernelGGL(adj_p2g_kernel, dim3(nblk(a.n, 128)), dim3(128), 0, st, a);
 }
