@@ -1380,9 +1380,10 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
     hipEventRecord(e1, s);
     tpairs.push_back({mode, e0, e1});
   }
-  // the deferred list is short (near-degenerate particles): a small grid keeps
-  // the usually empty pass cheap
-  hipLaunchKernelGGL(deferred_kernel, dim3(32), dim3(ENG_THREADS), 0, s, a);
+  // one workgroup per CU: the usual empty list costs each workgroup one load,
+  // and a violent step that defers tens of thousands of particles to FP64
+  // (seen with dt several times the stable step) is spread over the chip
+  hipLaunchKernelGGL(deferred_kernel, dim3(num_cus), dim3(ENG_THREADS), 0, s, a);
 }
 
 void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k) {
