@@ -812,8 +812,8 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       if (x0 == STAGE || x1 == STAGE || x2 == STAGE) cn[sn] = 0u;
     }
     // up to three chunks per source are loaded in one round trip (a block
-    // of ~20 particles per cell has three 512-particle chunks); the sums
-    // keep source-major, chunk-minor order
+    // of ~20 particles per cell has three 512-particle chunks) and summed
+    // source-major, chunk-minor
     float4 sv[3][8];
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn) {
@@ -831,22 +831,31 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
         q.z += sv[c][sn].z;
         q.w += sv[c][sn].w;
       }
-      // dense sources (many chunks per block): eight loads in flight per
-      // round, summed in the same chunk order
-      for (uint32_t c = 3; c < cn[sn]; c += 8) {
-        float4 v8[8];
+    }
+    // dense sources (more than three chunks): each round loads the next two
+    // chunks of every source together, so a block's dependent rounds follow
+    // its densest source rather than the sum over sources; the sums keep a
+    // fixed round-major order
+    uint32_t cmax = 0;
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
-          v8[u] = c + u < cn[sn] ? a.stage[(size_t)(c0[sn] + c + u) * STAGE_NODES + ai[sn]]
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int sn = 0; sn < 8; ++sn) cmax = cn[sn] > cmax ? cn[sn] : cmax;
+    for (uint32_t c = 3; c < cmax; c += 2) {
+      float4 v2[8][2];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          q.x += v8[u].x;
-          q.y += v8[u].y;
-          q.z += v8[u].z;
-          q.w += v8[u].w;
+      for (int sn = 0; sn < 8; ++sn)
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+          v2[sn][u] = c + u < cn[sn] ? a.stage[(size_t)(c0[sn] + c + u) * STAGE_NODES + ai[sn]]
+                                     : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int sn = 0; sn < 8; ++sn)
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+          q.x += v2[sn][u].x;
+          q.y += v2[sn][u].y;
+          q.z += v2[sn][u].z;
+          q.w += v2[sn][u].w;
         }
-      }
     }
     if (a.pm) a.pm[addr] = q;
     const float m = q.w;
