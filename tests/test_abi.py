@@ -47,6 +47,15 @@ def test_argument_errors_do_not_touch_the_gpu():
     assert L.mpm_collide_planes(None, _lib.MPM_F32, 8, 0.125, 17, pts, nrm, 0, ct.c_void_p(16)) == _lib.MPM_ERR_ARG
     assert L.mpm_collide_planes(None, _lib.MPM_F32, 8, 0.125, 1, pts, nrm, 7, ct.c_void_p(16)) == _lib.MPM_ERR_ARG
     assert L.mpm_ctx_set_planes(None, 0, 1, pts, nrm, 0) == _lib.MPM_ERR_ARG
+    # replayed-grid adjoint: null context, bad parameters, null grids
+    assert L.mpm_run_states_grids(None, 1, None, None, None, None, None, None) == _lib.MPM_ERR_ARG
+    pr3 = _lib.Params(3, 16, 16.0, 1 / 16, 1e-4, 1.0, -9.8, 0, 3, 0, 1)
+    parts = _lib.Particles(0, _lib.MPM_F32, 0, None, None, None, None, None, None, None, None, None)
+    assert L.mpm_adjoint_step_grids(None, ct.byref(bad), ct.byref(parts), 1.0, None, ct.c_void_p(16), None, None,
+                                    None, None, None, None, None) == _lib.MPM_ERR_ARG
+    assert L.mpm_adjoint_step_grids(None, ct.byref(pr3), ct.byref(parts), 1.0, None, ct.c_void_p(16), None, None,
+                                    None, None, None, None, None) == _lib.MPM_ERR_ARG
+    assert "null grid" in _lib.last_error()
     with pytest.raises(ValueError):
         _lib.check(_lib.MPM_ERR_ARG)
     with pytest.raises(RuntimeError):
