@@ -58,25 +58,49 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(state, params, budget_s):
-    """The C restatement of the reference step on the host (one thread)."""
+def host_threads():
+    """Threads this process may run on (affinity), capped by OMP_NUM_THREADS."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(avail, int(env))) if env.isdigit() else max(1, avail)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(state, params, budget_s, nthreads=1):
+    """The C restatement of the reference step on the host, on `nthreads`
+    threads (OpenMP over particles, atomic grid adds, for more than one)."""
     from oracle import c_oracle
 
     cs = c_oracle.CState(state)
     t0 = time.perf_counter()
-    c_oracle.run(state, params, 1, nthreads=1, cstate=cs)
+    c_oracle.run(state, params, 1, nthreads=nthreads, cstate=cs)
     t1 = time.perf_counter() - t0
     steps = 1
-    extra = int(max(0, min(9, budget_s / max(t1, 1e-6) - 1)))
+    extra = int(max(0, min(29, budget_s / max(t1, 1e-6) - 1)))
     if extra:
         t0 = time.perf_counter()
-        c_oracle.run(state, params, extra, nthreads=1, cstate=cs)
+        c_oracle.run(state, params, extra, nthreads=nthreads, cstate=cs)
         t1 += time.perf_counter() - t0
         steps += extra
     n = len(state["x"])
-    return {"value": n * steps / t1, "unit": "particle-substeps/s", "cores": 1, "kind": "port",
-            "sample": f"C restatement of the reference step (FP64, one thread, reference loop order), "
-                      f"{steps} substep(s) of the full {n}-particle scene from t=0"}
+    how = "one thread" if nthreads == 1 else f"{nthreads} OpenMP threads"
+    return {"value": n * steps / t1, "unit": "particle-substeps/s", "cores": nthreads, "kind": "port",
+            "sample": f"C restatement of the reference step (FP64, {how}, reference loop order), "
+                      f"{steps} substep(s) of the full {n}-particle scene from t=0",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
 
 def batched_bench(steps, dev):
@@ -300,9 +324,12 @@ def main():
     corrector = corrector_bench(dev) if args.corrector else None
     adjoint = adjoint_bench(dev) if args.adjoint else None
 
-    cpu = None
+    cpu = cpu_omp = None
     if args.cpu_seconds > 0:
         cpu = cpu_baseline(st, pr, args.cpu_seconds)
+        nt = host_threads()
+        if nt > 1:
+            cpu_omp = cpu_baseline(st, pr, args.cpu_seconds, nthreads=nt)
 
     out = {
         "metric": METRIC,
@@ -337,6 +364,7 @@ def main():
         "corrector": corrector,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
+        "cpu_baseline_omp": cpu_omp,
     }
     print(json.dumps(out), flush=True)
 
