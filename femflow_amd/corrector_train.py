@@ -257,7 +257,7 @@ class CorrectedRunner:
     """Corrected big steps with the plain substep as fallback.
 
     Runs in chunks of `check_every` big steps.  After each chunk the state is
-    judged healthy when no stencil left the grid, every position is finite and
+    deemed healthy when no stencil left the grid, every position is finite and
     the largest particle speed stays below the grid's velocity clamp; an
     unhealthy chunk is discarded and rerun with factor * check_every plain
     substeps from its saved start state."""
@@ -271,11 +271,11 @@ class CorrectedRunner:
         self.check_every = check_every
         self.fallbacks = 0
 
-    def run(self, s: Dict[str, torch.Tensor], n_big: int, judge: bool = True) -> Dict[str, torch.Tensor]:
+    def run(self, s: Dict[str, torch.Tensor], n_big: int, check: bool = True) -> Dict[str, torch.Tensor]:
         cfg = self.cfg
         n = s["x"].shape[0]
         big = make_engine(cfg, self.volume, cfg.dt_big, cfg.gravity, self.device, n)
-        plain = make_engine(cfg, self.volume, cfg.dt, cfg.gravity, self.device, n) if judge else None
+        plain = make_engine(cfg, self.volume, cfg.dt, cfg.gravity, self.device, n) if check else None
         if self.net is not None:
             big.set_corrector(self.net.params, cfg.m_ref, cfg.v_ref, enable=True)
         cur = s
@@ -285,7 +285,7 @@ class CorrectedRunner:
             _upload(big, cur)
             while done < n_big:
                 k = min(self.check_every, n_big - done)
-                if not judge:
+                if not check:
                     big.run(k, check_oob=False)
                     done += k
                     continue
@@ -333,7 +333,7 @@ def one_step_gain(cfg: CorrectorConfig, net: CorrectorMLP, device, seeds=(901, 9
         eng.close()
         disp = (ref - s["x"]).norm()
         for tag, nn in (("without", None), ("with", net)):
-            out = CorrectedRunner(cfg, nn, s["volume"], device).run(s, 1, judge=False)
+            out = CorrectedRunner(cfg, nn, s["volume"], device).run(s, 1, check=False)
             errs[tag].append(float((out["x"] - ref).norm() / disp))
     return {k: float(np.mean(v)) for k, v in errs.items()}
 

@@ -3,7 +3,7 @@
 err(field) = ||a - b||_inf / max(||b||_inf, floor).  The floors are the
 magnitude of the terms each field is summed from, so that a field that is
 zero up to cancellation (C and the off-diagonals of F in free fall, for
-example) is judged against the scale its rounding error actually has:
+example) is measured against the scale its rounding error actually has:
   x, F, Jp : floor 1e-12 (their norm is O(1))
   v        : floor 1e-6 (m/s)
   C        : floor 4 inv_dx ||v||_inf (C = 4 inv_dx sum w g (ijk - fx)^T)

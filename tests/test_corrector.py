@@ -252,7 +252,7 @@ def test_trained_corrector_beats_uncorrected_big_step(gpu):
         ref = eng.positions().clone()
         eng.close()
         for tag, nn in (("plain", None), ("corr", net)):
-            out = CT.CorrectedRunner(cfg, nn, s["volume"], gpu).run(s, 1, judge=False)
+            out = CT.CorrectedRunner(cfg, nn, s["volume"], gpu).run(s, 1, check=False)
             disp = (ref - s["x"]).norm()
             errs[tag].append(float((out["x"] - ref).norm() / disp))
     gen.close()
