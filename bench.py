@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--chunk", type=int, default=512)
     ap.add_argument("--sort-every", type=int, default=32)
     ap.add_argument("--grid-mult", type=int, default=8)
+    ap.add_argument("--spread", type=int, default=9, choices=(3, 9),
+                    help="scatter start offsets per cell (see Engine.tune)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="budget of the CPU-baseline sample (0 skips it)")
@@ -266,7 +268,7 @@ def main():
     eng = Engine([SceneSpec(res=pr["res"], dt=pr["dt"], gravity=pr["gravity"], volume=pr["volume"])], mats,
                  n_max=n, device=dev)
     eng.tune(chunk=args.chunk, sort_every=args.sort_every, grid_mult=args.grid_mult,
-             use_graph=not args.no_graph)
+             use_graph=not args.no_graph, spread=args.spread)
     eng.upload(x, v, F, C, Jp, mass, mu0, lam0, kind)
     eng.run(args.warmup, check_oob=False)
     torch.cuda.synchronize()
@@ -348,7 +350,7 @@ def main():
             "workload": "3D MLS-MPM substep, 2^20 particles in three seeded cubes (elastic fixed-corotated, "
                         "snow, weakly compressible water), 128^3 grid, dt=5e-5",
             "particles": n, "grid": f"{args.res + 1}^3 nodes", "chunk": args.chunk,
-            "sort_every": args.sort_every, "hip_graph": not args.no_graph,
+            "sort_every": args.sort_every, "spread": args.spread, "hip_graph": not args.no_graph,
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -23,7 +23,7 @@ MPM_KEY_BLOCKED, MPM_KEY_ROWMAJOR = 0, 1
 # every symbol include/mpm355.h declares (checked by the CPU test suite)
 EXPORTS = (
     "mpm_last_error_string", "mpm_version", "mpm_device_count", "mpm_p2g", "mpm_grid_op",
-    "mpm_g2p", "mpm_ctx_create", "mpm_ctx_destroy", "mpm_ctx_set_stream", "mpm_ctx_tune",
+    "mpm_g2p", "mpm_ctx_create", "mpm_ctx_destroy", "mpm_ctx_set_stream", "mpm_ctx_tune", "mpm_ctx_set_spread",
     "mpm_ctx_kernel_timing", "mpm_ctx_kernel_times", "mpm_state_upload", "mpm_state_download",
     "mpm_run", "mpm_run_frames", "mpm_run_states", "mpm_run_states_grids", "mpm_state_positions", "mpm_substep", "mpm_ctx_oob_count", "mpm_ctx_fallback_count",
     "mpm_particle_sums", "mpm_bin", "mpm_dense_workspace_bytes", "mpm_dense_substep", "mpm_adjoint_step",
@@ -95,6 +95,7 @@ def load():
             "mpm_ctx_destroy": ([vp], i32),
             "mpm_ctx_set_stream": ([vp, vp], i32),
             "mpm_ctx_tune": ([vp, i32, i32, i32, i32], i32),
+            "mpm_ctx_set_spread": ([vp, i32], i32),
             "mpm_ctx_kernel_timing": ([vp, i32], i32),
             "mpm_ctx_kernel_times": ([vp, ct.POINTER(dbl), ct.POINTER(i64)], i32),
             "mpm_state_upload": ([vp, P, vp], i32),

@@ -2,7 +2,7 @@
 
 For a loaded ``Engine`` it times a short substep sequence for every candidate
 (particles per work item, persistent workgroups per CU, substeps between
-rebinning) and keeps the fastest.  The engine state is saved before and
+rebinning, scatter spread) and keeps the fastest.  The engine state is saved before and
 restored after the search, so tuning does not advance the simulation.
 Results are cached in a JSON file keyed by the workload's signature (device
 name, particle count, scene resolutions) so later runs skip the search.
@@ -22,6 +22,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "femflow_amd", "
 CHUNKS = (256, 512, 1024)
 GRID_MULTS = (4, 8)
 SORT_EVERY = (16, 32)
+SPREADS = (3, 9)
 
 
 def signature(engine) -> str:
@@ -56,9 +57,10 @@ def _time(engine, steps: int, reps: int) -> float:
 
 
 def autotune(engine, restore: Tuple, steps: int = 32, reps: int = 2,
-             candidates: Optional[Iterable[Tuple[int, int, int]]] = None,
+             candidates: Optional[Iterable[Tuple[int, ...]]] = None,
              cache_path: Optional[str] = DEFAULT_CACHE, verbose: bool = False) -> Dict:
-    """Pick (chunk, grid_mult, sort_every) for ``engine``.
+    """Pick (chunk, grid_mult, sort_every, spread) for ``engine``; a candidate
+    given as (chunk, grid_mult, sort_every) takes spread 9.
 
     ``restore`` = (mass, mu0, lam0, material, scene) tensors that were passed to
     ``upload`` (the engine keeps only its binned copy), used to put the state
@@ -75,17 +77,19 @@ def autotune(engine, restore: Tuple, steps: int = 32, reps: int = 2,
             return dict(cached, cached=True)
     snap = _snapshot(engine)
     mass, mu0, lam0, material, scene = restore
-    cands = list(candidates) if candidates is not None else list(itertools.product(CHUNKS, GRID_MULTS, SORT_EVERY))
+    cands = (list(candidates) if candidates is not None
+             else list(itertools.product(CHUNKS, GRID_MULTS, SORT_EVERY, SPREADS)))
+    cands = [tuple(c) + (9,) * (4 - len(c)) for c in cands]
     table = []
     best = None
-    for chunk, gm, se in cands:
-        engine.tune(chunk=chunk, grid_mult=gm, sort_every=se)
+    for chunk, gm, se, sp in cands:
+        engine.tune(chunk=chunk, grid_mult=gm, sort_every=se, spread=sp)
         t = _time(engine, max(steps, se), reps)
-        table.append({"chunk": chunk, "grid_mult": gm, "sort_every": se, "ms_per_substep": t})
+        table.append({"chunk": chunk, "grid_mult": gm, "sort_every": se, "spread": sp, "ms_per_substep": t})
         if verbose:
-            print("autotune", chunk, gm, se, f"{t:.4f} ms", flush=True)
+            print("autotune", chunk, gm, se, sp, f"{t:.4f} ms", flush=True)
         if best is None or t < best[0]:
-            best = (t, dict(chunk=chunk, grid_mult=gm, sort_every=se))
+            best = (t, dict(chunk=chunk, grid_mult=gm, sort_every=se, spread=sp))
         engine.upload(snap["x"], snap["v"], snap["F"], snap["C"], snap["Jp"], mass, mu0, lam0, material, scene)
     engine.tune(**best[1])
     result = {"config": best[1], "ms_per_substep": best[0], "table": table}

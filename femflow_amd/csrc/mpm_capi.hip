@@ -355,6 +355,21 @@ int32_t mpm_ctx_tune(mpm_ctx* ctx, int32_t chunk, int32_t sort_every, int32_t gr
   return MPM_OK;
 }
 
+int32_t mpm_ctx_set_spread(mpm_ctx* ctx, int32_t spread) {
+  if (!ctx || (spread != 3 && spread != 9)) {
+    set_error("mpm_ctx_set_spread: spread must be 3 or 9");
+    return MPM_ERR_ARG;
+  }
+  mpm::Engine& e = ctx->eng;
+  if (e.spread != spread) {  // captured graphs hold the other kernel
+    hipStreamSynchronize(e.own_stream);
+    for (auto& kv : e.graphs) hipGraphExecDestroy(kv.second);
+    e.graphs.clear();
+  }
+  e.spread = spread;
+  return MPM_OK;
+}
+
 int32_t mpm_ctx_kernel_timing(mpm_ctx* ctx, int32_t enable) {
   if (!ctx) {
     set_error("mpm_ctx_kernel_timing: null context");

@@ -51,6 +51,7 @@ struct Engine {
   int64_t n_max = 0, n = 0, total_blocks = 0;
   int nmat = 1;
   int chunk = 512, sort_every = 8, grid_mult = 4;
+  int spread = 9;  // scatter start offsets per cell (3 or 9, see transfer_kernel)
   bool use_graph = true, timing = false;
   std::vector<SceneDev> scenes_host;
   // streams and graphs

@@ -375,8 +375,16 @@ __device__ __forceinline__ int round_fixed(float w, float q) {
   return r;
 }
 
-template <int MODE, int P>
+// SPREAD (3 or 9): how many different starting nodes the lanes that share a
+// cell use in the scatter.  3 rotates the node slab (q mod 3); 9 also
+// rotates the row inside the slab ((q mod 9) / 3), which spreads dense
+// cells' atomics over three times as many LDS words at the cost of a
+// runtime row index (a few VALU per slab).  Measured at 2^19 particles:
+// equal at 16 particles per cell, 25% / 34% faster at 128 / 1,157, and 1.5%
+// faster on the three-cube 2^20 scene (~21 per cell), so 9 is the default.
+template <int MODE, int P, int SPREAD>
 __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
+  static_assert(SPREAD == 3 || SPREAD == 9, "scatter spread is 3 or 9");
   // The velocity arena (read by the G2P half) and the accumulation arena
   // (written by the P2G half after every gather of the chunk is done) share
   // one LDS region.
@@ -560,7 +568,21 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
           A2s[d] = pt_s[8 + 3 * d][qs] * scl[d];
         }
         const float ms = pt_s[15][qs] * scl[3];
-        const int i0 = q % 3;
+        const int i0 = SPREAD == 9 ? (q % 9) % 3 : q % 3;
+        // row order of the slab (identity for SPREAD 3, where it folds to
+        // the compile-time rows)
+        float w1r[3], fjr[3];
+        int jor[3];
+        {
+          const int j0 = SPREAD == 9 ? (q % 9) / 3 : 0;
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) {
+            const int j = SPREAD == 9 ? (j0 + jj >= 3 ? j0 + jj - 3 : j0 + jj) : jj;
+            w1r[jj] = SPREAD == 9 ? (j == 0 ? w1[0] : (j == 1 ? w1[1] : w1[2])) : w1[jj];
+            fjr[jj] = (float)j;
+            jor[jj] = j * ACC_S1;
+          }
+        }
 #pragma unroll 1
         for (int sl = 0; sl < 3; ++sl) {
           const int i = i0 + sl >= 3 ? i0 + sl - 3 : i0 + sl;
@@ -571,15 +593,15 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) qi[d] = q0s[d] + A0s[d] * fi;
 #pragma unroll
-          for (int j = 0; j < 3; ++j) {
-            const float wij = wi * w1[j];
+          for (int jj = 0; jj < 3; ++jj) {
+            const float wij = wi * w1r[jj];
             float qij[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) qij[d] = qi[d] + A1s[d] * (float)j;
+            for (int d = 0; d < 3; ++d) qij[d] = qi[d] + A1s[d] * fjr[jj];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
               const float wt = wij * w2[k];
-              const int off = ai + j * ACC_S1 + k;
+              const int off = ai + jor[jj] + k;
               // rounded to nearest (fma with 0.5, floor conversion: the
               // same two instructions as a product and a truncation)
               const int c0 = round_fixed(wt, qij[0] + A2s[0] * (float)k);
@@ -1369,8 +1391,13 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
     e1 = tevent();
     hipEventRecord(e0, s);
   }
-#define MPM_LAUNCH_T(M, PV) \
-  hipLaunchKernelGGL((transfer_kernel<M, PV>), dim3(g), dim3(ENG_THREADS), 0, s, a)
+#define MPM_LAUNCH_T(M, PV)                                                                          \
+  do {                                                                                               \
+    if (M != 2 && spread == 9)                                                                       \
+      hipLaunchKernelGGL((transfer_kernel<M, PV, (M == 2 ? 3 : 9)>), dim3(g), dim3(ENG_THREADS), 0, s, a); \
+    else                                                                                             \
+      hipLaunchKernelGGL((transfer_kernel<M, PV, 3>), dim3(g), dim3(ENG_THREADS), 0, s, a);        \
+  } while (0)
   if (chunk == 256) {
     if (mode == 0) MPM_LAUNCH_T(0, 1);
     else if (mode == 1) MPM_LAUNCH_T(1, 1);

@@ -84,13 +84,18 @@ class Engine:
     def _sync_stream(self):
         _lib.check(self._L.mpm_ctx_set_stream(self._h, self._stream()), "set_stream")
 
-    def tune(self, chunk: int = 512, sort_every: int = 8, grid_mult: int = 4, use_graph: bool = True):
+    def tune(self, chunk: int = 512, sort_every: int = 8, grid_mult: int = 4, use_graph: bool = True,
+             spread: int = 9):
         """chunk: binned particles per work item (256, 512, 1024); sort_every:
         substeps between rebinning; grid_mult: persistent workgroups per CU;
-        use_graph: replay the substep sequence as a captured hipGraph."""
-        self.config = dict(chunk=chunk, sort_every=sort_every, grid_mult=grid_mult, use_graph=use_graph)
+        use_graph: replay the substep sequence as a captured hipGraph;
+        spread: scatter start offsets per cell (9, or 3: the narrower
+        rotation, slower where many particles share a cell)."""
+        self.config = dict(chunk=chunk, sort_every=sort_every, grid_mult=grid_mult, use_graph=use_graph,
+                           spread=spread)
         _lib.check(self._L.mpm_ctx_tune(self._h, chunk, sort_every, grid_mult, 1 if use_graph else 0),
                    "tune")
+        _lib.check(self._L.mpm_ctx_set_spread(self._h, int(spread)), "tune")
 
     def _particles(self, x, v, F, C, Jp, mass=None, mu0=None, lam0=None, material=None):
         def ptr(t):

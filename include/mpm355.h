@@ -151,6 +151,11 @@ int32_t mpm_ctx_set_stream(mpm_ctx* ctx, void* stream);
  * use_graph = capture and replay the substep sequence as a hipGraph. */
 int32_t mpm_ctx_tune(mpm_ctx* ctx, int32_t chunk, int32_t sort_every, int32_t grid_mult,
                      int32_t use_graph);
+/* Scatter spread (3 or 9, default 9): the number of different starting nodes
+ * that particles sharing a cell use in the P2G scatter; 9 keeps cells with
+ * hundreds of particles as fast as sparse ones.  Results agree to rounding.
+ * No reference counterpart (a tunable of this implementation). */
+int32_t mpm_ctx_set_spread(mpm_ctx* ctx, int32_t spread);
 
 /* Per-kernel timing with HIP events on the engine's stream (eager launches
  * only; graphs are not used while enabled).  mpm_ctx_kernel_times returns

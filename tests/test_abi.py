@@ -38,6 +38,7 @@ def test_argument_errors_do_not_touch_the_gpu():
     assert L.mpm_run(None, 1, None) == _lib.MPM_ERR_ARG
     assert "null context" in _lib.last_error()
     assert L.mpm_ctx_tune(None, 512, 1, 1, 1) == _lib.MPM_ERR_ARG
+    assert L.mpm_ctx_set_spread(None, 3) == _lib.MPM_ERR_ARG
     bad = _lib.Params(4, 16, 16.0, 1 / 16, 1e-4, 1.0, -9.8, 0, 3, 0, 1)
     assert L.mpm_grid_op(None, ct.byref(bad), _lib.MPM_F32, None, None) == _lib.MPM_ERR_ARG
     # plane colliders: null grid, too many planes, bad quirks mode, null context

@@ -58,12 +58,12 @@ def test_autotune_restores_state_and_caches(gpu, tmp_path):
     run = BatchedRun(work, device=gpu)
     before = run.download()
     cache = tmp_path / "tune.json"
-    res = run.autotune(steps=8, reps=1, candidates=[(256, 4, 8), (512, 8, 8)], cache_path=str(cache))
+    res = run.autotune(steps=8, reps=1, candidates=[(256, 4, 8, 3), (512, 8, 8)], cache_path=str(cache))
     after = run.download()
     for k in before:
         assert torch.equal(before[k], after[k]), k
-    assert res["config"] in ({"chunk": 256, "grid_mult": 4, "sort_every": 8},
-                             {"chunk": 512, "grid_mult": 8, "sort_every": 8})
+    assert res["config"] in ({"chunk": 256, "grid_mult": 4, "sort_every": 8, "spread": 3},
+                             {"chunk": 512, "grid_mult": 8, "sort_every": 8, "spread": 9})
     data = json.loads(cache.read_text())
     assert len(data) == 1
     again = run.autotune(cache_path=str(cache))

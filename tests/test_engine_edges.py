@@ -34,8 +34,8 @@ def test_engine_empty_state(gpu):
     eng.close()
 
 
-@pytest.mark.parametrize("chunk", [256, 512])
-def test_engine_thousands_in_one_cell(gpu, chunk):
+@pytest.mark.parametrize("chunk,spread", [(256, 3), (512, 3), (512, 9)])
+def test_engine_thousands_in_one_cell(gpu, chunk, spread):
     res = 32
     elastic = (O.MATERIALS_C2[0],)
     # 6000 particles in one cell plus a sparse cube around it
@@ -47,13 +47,13 @@ def test_engine_thousands_in_one_cell(gpu, chunk):
         st[k] = np.concatenate([st[k], st2[k]])
     ts = to_torch_state(st, gpu)
     ref = oracle_copy_from(ts, st)
-    eng = engine_for(st, pr, gpu, sort_every=2, chunk=chunk, use_graph=True)
+    eng = engine_for(st, pr, gpu, sort_every=2, chunk=chunk, use_graph=True, spread=spread)
     nsteps = 4
     out = engine_roundtrip(eng, ts, nsteps)
     for _ in range(nsteps):
         O.substep(ref, pr)
     errs = field_errors(out, ref, pr["inv_dx"])
-    _say("one cell", chunk, errs)
+    _say("one cell", chunk, spread, errs)
     for k in ("x", "v", "F"):
         assert errs[k] < 1e-5, errs
     assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs

@@ -223,19 +223,21 @@ def test_engine_one_substep_three_materials(gpu, quirks):
     eng.close()
 
 
-@pytest.mark.parametrize("sort_every,use_graph,chunk", [(1, False, 512), (4, False, 256), (4, True, 512),
-                                                         (8, True, 1024), (16, True, 512)])
-def test_engine_fused_multistep(gpu, sort_every, use_graph, chunk):
+@pytest.mark.parametrize("sort_every,use_graph,chunk,spread", [(1, False, 512, 3), (4, False, 256, 3),
+                                                                (4, True, 512, 3), (8, True, 1024, 3),
+                                                                (16, True, 512, 3), (4, True, 512, 9),
+                                                                (8, False, 1024, 9)])
+def test_engine_fused_multistep(gpu, sort_every, use_graph, chunk, spread):
     st, pr = O.scene_3d_cubes(n=30000, res=64)
     ts = to_torch_state(st, gpu)
     ref = oracle_copy_from(ts, st)
-    eng = engine_for(st, pr, gpu, sort_every=sort_every, use_graph=use_graph, chunk=chunk)
+    eng = engine_for(st, pr, gpu, sort_every=sort_every, use_graph=use_graph, chunk=chunk, spread=spread)
     nsteps = 10
     out = engine_roundtrip(eng, ts, nsteps)
     for _ in range(nsteps):
         O.substep(ref, pr)
     errs = field_errors(out, ref, pr["inv_dx"])
-    _say("engine fused", sort_every, use_graph, chunk, errs)
+    _say("engine fused", sort_every, use_graph, chunk, spread, errs)
     for k in ("x", "v", "F"):
         assert errs[k] < 1e-4, errs
     assert errs["Jp"] < 1e-4, errs
