@@ -21,6 +21,13 @@ __device__ __forceinline__ float weight_at(float fx, int i) {
   return i == 1 ? 0.75f - t * t : 0.5f * t * t;
 }
 
+// Velocity arena (G2P): node (a0, a1, a2) of the 8x8x8 arena at float4
+// a0*81 + a1*9 + a2.  The plain 64/8/1 layout puts cells that differ in a0
+// or in a1 by two at multiples of 256 bytes, on the same LDS banks; the odd
+// strides spread a block's 64 cells over all of them (one-to-one on the
+// arena, and it stays inside the accumulation arena it shares LDS with).
+constexpr int VEL_S0 = 81, VEL_S1 = 9, VEL_NODES = 7 * VEL_S0 + 7 * VEL_S1 + 7 + 1;
+
 // v = sum w g; C = 4 inv_dx sum w g (ijk - fx)^T  (three_d/g2p.py:29-46),
 // accumulated as C[:, a] = 4 inv_dx (sum w g ijk_a - fx_a v).  The sums are
 // taken axis by axis (w = wi wj wk): per row (i, j) the k-sums
@@ -56,7 +63,7 @@ __device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int aba
     const float wi = weight_at(fx[0], i);
     size_t Xi = 0;
     if (!FROM_LDS) Xi = i == 0 ? X[0] : (i == 1 ? X[1] : X[2]);
-    const int ai = abase + i * 64;
+    const int ai = abase + i * VEL_S0;
     float Vi[3] = {0.f, 0.f, 0.f}, Tj[3] = {0.f, 0.f, 0.f}, Tk[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
@@ -64,7 +71,7 @@ __device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int aba
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         float4 g;
-        if (FROM_LDS) g = vel_s[ai + j * 8 + k];
+        if (FROM_LDS) g = vel_s[ai + j * VEL_S1 + k];
         else g = vel[gbase + Xi + Y[j] + Z[k]];
         const float gg[3] = {g.x, g.y, g.z};
 #pragma unroll
@@ -336,6 +343,8 @@ constexpr bool acc_layout_one_to_one() {
   return true;
 }
 static_assert(acc_layout_one_to_one(), "accumulation arena layout must be one-to-one on 8x8x8 nodes");
+static_assert(VEL_S1 >= 8 && VEL_S0 >= 7 * VEL_S1 + 8, "velocity arena layout must be one-to-one");
+static_assert(4 * VEL_NODES <= 4 * ACC_NODES, "velocity arena must fit the LDS it shares");
 
 // Two int32 fixed-point values in one 64-bit word whose 64-bit sum carries
 // both sums exactly: word = hi * 2^32 + lo (lo sign-extended).  Valid while
@@ -388,7 +397,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   // The velocity arena (read by the G2P half) and the accumulation arena
   // (written by the P2G half after every gather of the chunk is done) share
   // one LDS region.
-  constexpr int VEL_WORDS = MODE == 0 ? 4 : 4 * ARENA_NODES;
+  constexpr int VEL_WORDS = MODE == 0 ? 4 : 4 * VEL_NODES;
   constexpr int ACC_WORDS = MODE == 2 ? 4 : 4 * ACC_NODES;
   __shared__ __align__(16) uint32_t arena_s[VEL_WORDS > ACC_WORDS ? VEL_WORDS : ACC_WORDS];
   float4* const vel_s = reinterpret_cast<float4*>(arena_s);
@@ -428,7 +437,8 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
       for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
         const int gi = O0 + (e >> 6), gj = O1 + ((e >> 3) & 7), gk = O2 + (e & 7);
         const bool in = gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res;
-        vel_s[e] = in ? a.vel[node_addr(sc, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        vel_s[(e >> 6) * VEL_S0 + ((e >> 3) & 7) * VEL_S1 + (e & 7)] =
+            in ? a.vel[node_addr(sc, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
     if (t < 27) sub_active[t] = 0;
@@ -465,7 +475,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
         const int a0 = base[0] - O0, a1 = base[1] - O1, a2 = base[2] - O2;
         if (a0 >= 0 && a0 <= 5 && a1 >= 0 && a1 <= 5 && a2 >= 0 && a2 <= 5)
-          gather<true>(vel_s, (a0 * 8 + a1) * 8 + a2, a.vel, sc, base, w, fx, v, C);
+          gather<true>(vel_s, a0 * VEL_S0 + a1 * VEL_S1 + a2, a.vel, sc, base, w, fx, v, C);
         else
           gather<false>(vel_s, 0, a.vel, sc, base, w, fx, v, C);
         const bool ok = advance(sc, mp, x, v, C, F, Jp);
