@@ -407,12 +407,14 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   __shared__ ChunkScale cs;
   __shared__ uint32_t sub_active[27];
   __shared__ uint32_t wg_oob, wg_fb;
-  // P2G term table (fx[3], q0[3], Adx[9], m per chunk entry) and the arena
-  // cell of each entry (-1: not in the table).  Entry q sits at slot
-  // q + q/16, which spreads the scatter phase's strided reads over the banks.
+  // P2G term table, four float4 per chunk entry, (fx0, fx1, fx2, m),
+  // (q0[0..2], Adx0), (Adx1..4), (Adx5..8), read and written 16 bytes at a
+  // time, and the arena cell of each entry (-1: not in the table).  Entry q
+  // sits at slot q + q/16, which spreads the scatter phase's strided reads
+  // over the banks.
   constexpr int TAB = MODE == 2 ? 1 : P * ENG_THREADS;
   constexpr int TABS = TAB + TAB / 16;
-  __shared__ float pt_s[16][TABS];
+  __shared__ float4 pt4[4][TABS];
   __shared__ int pb_s[TABS];
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
@@ -515,14 +517,10 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
           if (finite) {
 #pragma unroll
             for (int d = 0; d < 4; ++d) bsum[d] += b[d];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-              pt_s[d][qs] = fx[d];
-              pt_s[3 + d][qs] = tm.q0[d];
-            }
-#pragma unroll
-            for (int e = 0; e < 9; ++e) pt_s[6 + e][qs] = tm.Adx[e];
-            pt_s[15][qs] = tm.m;
+            pt4[0][qs] = make_float4(fx[0], fx[1], fx[2], tm.m);
+            pt4[1][qs] = make_float4(tm.q0[0], tm.q0[1], tm.q0[2], tm.Adx[0]);
+            pt4[2][qs] = make_float4(tm.Adx[1], tm.Adx[2], tm.Adx[3], tm.Adx[4]);
+            pt4[3][qs] = make_float4(tm.Adx[5], tm.Adx[6], tm.Adx[7], tm.Adx[8]);
             pb_s[qs] = a0 * ACC_S0 + a1 * ACC_S1 + a2;
           } else {  // NaN/Inf terms keep their value through the float path
             inside = false;
@@ -565,19 +563,25 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         const int qs = q + (q >> 4);
         const int pb = pb_s[qs];
         if (pb < 0) continue;
-        const float fx0 = pt_s[0][qs];
+        const float4 t0 = pt4[0][qs], t1 = pt4[1][qs], t2 = pt4[2][qs], t3 = pt4[3][qs];
+        const float fx0 = t0.x;
         float w1[3], w2[3];
-        bspline(pt_s[1][qs], w1);
-        bspline(pt_s[2][qs], w2);
+        bspline(t0.y, w1);
+        bspline(t0.z, w2);
+        // Adx row-major: A0s[d] = Adx[3d], A1s[d] = Adx[3d+1], A2s[d] = Adx[3d+2]
+        const float q0r[3] = {t1.x, t1.y, t1.z};
+        const float A0r[3] = {t1.w, t2.z, t3.y};
+        const float A1r[3] = {t2.x, t2.w, t3.z};
+        const float A2r[3] = {t2.y, t3.x, t3.w};
         float q0s[3], A0s[3], A1s[3], A2s[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-          q0s[d] = pt_s[3 + d][qs] * scl[d];
-          A0s[d] = pt_s[6 + 3 * d][qs] * scl[d];
-          A1s[d] = pt_s[7 + 3 * d][qs] * scl[d];
-          A2s[d] = pt_s[8 + 3 * d][qs] * scl[d];
+          q0s[d] = q0r[d] * scl[d];
+          A0s[d] = A0r[d] * scl[d];
+          A1s[d] = A1r[d] * scl[d];
+          A2s[d] = A2r[d] * scl[d];
         }
-        const float ms = pt_s[15][qs] * scl[3];
+        const float ms = t0.w * scl[3];
         const int i0 = SPREAD == 9 ? (q % 9) % 3 : q % 3;
         // row order of the slab (identity for SPREAD 3, where it folds to
         // the compile-time rows)
