@@ -597,7 +597,9 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
             jor[jj] = j * ACC_S1;
           }
         }
-#pragma unroll 1
+        // the three slabs unrolled: the scheduler interleaves one slab's
+        // conversions with the previous slab's LDS atomics (same VGPRs)
+#pragma unroll
         for (int sl = 0; sl < 3; ++sl) {
           const int i = i0 + sl >= 3 ? i0 + sl - 3 : i0 + sl;
           const float wi = weight_at(fx0, i);
