@@ -10,7 +10,10 @@ keep a checkpoint of the particle state every ``cfg.checkpoint_every``
 substeps; the backward replays one checkpoint interval at a time on the
 engine, keeping that interval's per-substep states (24 floats per particle
 per substep) and dense forward grids (7 floats per node per substep, from
-``mpm_run_states_grids``), and reverses it with ``mpm_adjoint_step_grids``.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
+``mpm_run_states_grids``), and reverses it with ``mpm_adjoint_step_grids``.
+When the record of the whole rollout fits in ``record_budget_gib`` (10 GiB
+for 2^17 particles, a 64^3 grid and 500 substeps), the forward itself keeps
+it and the backward replays nothing.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
 or scalar); mass is a constant.
 
 The reference has no gradients; the forward it differentiates is its step
@@ -38,6 +41,11 @@ class RolloutConfig:
     gravity: float = -9.8
     hardening: float = 1.0
     checkpoint_every: int = 50
+    # when the whole forward record (24 floats per particle and 7 per grid
+    # node, per substep) fits in this many GiB of device memory, the forward
+    # keeps it and the backward replays nothing; otherwise checkpoints every
+    # `checkpoint_every` substeps and one replay per interval
+    record_budget_gib: float = 32.0
     boundary: str = "reference"
     bound: int = 3
     quirks: str = "reference"
@@ -178,13 +186,22 @@ class _Rollout(torch.autograd.Function):
         F, C = _f32(F0, dev).clone(), _f32(C0, dev).clone()
         K = max(1, int(cfg.checkpoint_every))
         ckpts = []
+        record = None
         fwd.load(x, v, F, C)
-        for s0 in range(0, cfg.nsteps, K):
-            ckpts.append((x, v, F, C) if s0 == 0 else fwd.state())
-            fwd.run(min(K, cfg.nsteps - s0))
-        x, v, _, _ = fwd.state()
+        per_step = 4 * (24 * n + 7 * (cfg.res + 1) ** 3)
+        if cfg.nsteps > 0 and per_step * cfg.nsteps <= cfg.record_budget_gib * 2 ** 30:
+            # one recorded forward: every substep's state and grids stay on
+            # the device for the backward
+            after, grids = fwd.replay(cfg.nsteps)
+            record = ([(x, v, F, C)] + after[:-1], grids)
+            x, v = after[-1][0].clone(), after[-1][1].clone()
+        else:
+            for s0 in range(0, cfg.nsteps, K):
+                ckpts.append((x, v, F, C) if s0 == 0 else fwd.state())
+                fwd.run(min(K, cfg.nsteps - s0))
+            x, v, _, _ = fwd.state()
         fwd.check_oob()
-        ctx.stepper, ctx.fwd, ctx.ckpts, ctx.cfg = st, fwd, ckpts, cfg
+        ctx.stepper, ctx.fwd, ctx.ckpts, ctx.cfg, ctx.record = st, fwd, ckpts, cfg, record
         ctx.mats = (mass_, mu_, lam_)
         ctx.shapes = (mu0.shape, lam0.shape, mu0.dtype, lam0.dtype)
         return x, v
@@ -203,6 +220,13 @@ class _Rollout(torch.autograd.Function):
         lamb = torch.zeros_like(lam_)
         K = max(1, int(cfg.checkpoint_every))
         fwd = ctx.fwd
+        if ctx.record is not None:
+            states, grids = ctx.record
+            for k in range(cfg.nsteps - 1, -1, -1):
+                x, v, F, C = states[k]
+                st.adjoint_grids(x, v, F, C, mass_, mu_, lam_, grids[k], xb, vb, Fb, Cb, mub, lamb)
+            ctx.record = None
+            del states, grids
         for seg in range(len(ctx.ckpts) - 1, -1, -1):
             s0 = seg * K
             s1 = min(s0 + K, cfg.nsteps)

@@ -173,14 +173,16 @@ def test_recorded_engine_grids_match_dense_grids(gpu):
 
 
 @pytest.mark.gpu
-def test_rollout_gradient_matches_fp64_reference(gpu):
+@pytest.mark.parametrize("budget", [32.0, 0.0])  # whole forward recorded / checkpoints and replays
+def test_rollout_gradient_matches_fp64_reference(gpu, budget):
     from femflow_amd.adjoint import RolloutConfig, rollout
 
     n, res, dt, steps = 1024, 32, 1e-4, 120
     sc = _cube_scene(n, res, 21, vscale=0.5, fnoise=0.0, cnoise=0.0)
     F0 = np.repeat(np.eye(3)[None], n, 0)
     C0 = np.zeros((n, 3, 3))
-    cfg = RolloutConfig(res=res, dt=dt, volume=sc["vol"], nsteps=steps, checkpoint_every=50)
+    cfg = RolloutConfig(res=res, dt=dt, volume=sc["vol"], nsteps=steps, checkpoint_every=50,
+                        record_budget_gib=budget)
     rng = np.random.default_rng(0)
     target = sc["x"].mean(0) + 0.02 * rng.standard_normal(3)
     f32 = lambda a: torch.tensor(np.asarray(a), dtype=torch.float32, device=gpu)  # noqa: E731
@@ -198,7 +200,7 @@ def test_rollout_gradient_matches_fp64_reference(gpu):
     print("rollout loss gpu", float(L), "fp64", float(Lr), flush=True)
     for name, g, rr in zip(("x0", "v0", "mu0", "lam0"), (gx, gv, gmu, glam), ref):
         e = _rel(g.cpu().numpy(), rr.numpy())
-        print(f"rollout d/d{name}: rel err {e:.2e}", flush=True)
+        print(f"rollout (budget {budget}) d/d{name}: rel err {e:.2e}", flush=True)
         assert e < 5e-3, name
 
 
