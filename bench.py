@@ -40,7 +40,7 @@ def parse():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--res", type=int, default=128)
     ap.add_argument("--chunk", type=int, default=512)
-    ap.add_argument("--sort-every", type=int, default=32)
+    ap.add_argument("--sort-every", type=int, default=48)
     ap.add_argument("--grid-mult", type=int, default=8)
     ap.add_argument("--spread", type=int, default=9, choices=(3, 9),
                     help="scatter start offsets per cell (see Engine.tune)")

@@ -21,7 +21,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "femflow_amd", "
 
 CHUNKS = (256, 512, 1024)
 GRID_MULTS = (4, 8)
-SORT_EVERY = (16, 32)
+SORT_EVERY = (16, 32, 48)
 SPREADS = (3, 9)
 
 
