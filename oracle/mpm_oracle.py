@@ -38,16 +38,27 @@ THETA_C_REF, THETA_S_REF = 2.5e-2, 7.5e-3  # three_d/g2p.py:50, two_d/g2p.py:323
 # Which oracle functions are pinned by what (checked in tests/test_oracle.py).
 PARITY = {
     "weights": "KAT test_utils.py:6-15; partition of unity; first/second moments",
-    "lame": "KAT test_particle_to_grid.py:30-38 (hardening at Jp=1); closed form",
+    "lame": "KAT test_particle_to_grid.py:30-38 (hardening at Jp=1); snow e=exp(h(1-Jp)) at Jp=0.9 and "
+            "1.05 against e^1 and e^-1/2 (tests/test_oracle_pins.py::test_pin_snow_hardening, which fails "
+            "for a negated exponent or a dropped h)",
     "stress_2d": "KAT test_particle_to_grid.py:41-50 off-diagonals; pure dilation closed form",
     "stress_3d": "rotation -> zero stress; pure dilation closed form; scipy polar",
     "polar_3d": "scipy.linalg.polar (independent library routine)",
-    "p2g_2d": "KAT ground_truth_grid.txt multiset; mass and momentum conservation",
-    "p2g_3d": "mass/momentum conservation; C++ loop restatement (oracle/mpm_oracle.c)",
+    "p2g_2d": "KAT ground_truth_grid.txt multiset; mass and momentum conservation; APIC affine "
+              "reproduction (node velocity a + B x_node, tests/test_oracle_pins.py)",
+    "p2g_3d": "mass/momentum conservation; APIC affine reproduction: particles on v = a + B x with C = B, "
+              "F = I give node velocity a + B x_node to 1e-12 and the G2P returns v, C "
+              "(tests/test_oracle_pins.py::test_pin_affine_scatter, which fails for a flipped dpos, a dropped "
+              "dx, a negated or transposed affine term); C++ loop restatement (oracle/mpm_oracle.c)",
     "grid_op_2d": "per-node closed form on a brute-force grid",
     "grid_op_3d": "per-node closed form on a brute-force grid",
     "g2p_2d": "KAT test_grid_to_particle.py:31-103 (v, x); linear field reproduction",
     "g2p_3d": "linear field reproduction (v = a + B x, C = B); C++ restatement",
+    "snow_plasticity": "F = s I: clamp to c I, Jp <- clip(Jp s^3 / (c^3 + 1e-10), 0.6, 20) in closed form "
+                       "(tests/test_oracle_pins.py::test_pin_snow_plasticity, which fails for an inverted "
+                       "ratio, swapped thetas, a wrong clip or a lost J_old); C restatement pinned the same way",
+    "water": "P F^T = K (J-1) J I and F <- cbrt(J) I in closed form (tests/test_oracle_pins.py::test_pin_water, "
+             "which fails for the all-entries broadcast, a dropped J, a lost cube root or kept shear)",
     "bin_keys": "brute-force integer arithmetic; truncation toward zero",
     "bin_sort": "numpy.argsort(kind='stable') (library routine)",
     "collide_planes": "closed-form node counts (axis-aligned textbook planes; a hand-counted "
