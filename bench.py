@@ -9,11 +9,16 @@ FP32, the reference's 3D walls.  A step is one substep of the whole scene
 resident in HBM.
 
 Prints one JSON line.  `value` = particle-substeps/s over the timed steps.
+The timed window starts at a rebin (the engine rebins every `sort_every`
+substeps, counting across calls), and every graph it replays was captured
+during the warm-up, so the window holds ceil(steps / sort_every) rebins and
+no capture; `steady_state` times whole rebin periods for comparison.
 `roofline` is for the dominant kernel (the fused G2P+P2G transfer): its
-algorithmic bytes (212 B per particle-substep plus 28 B per active grid node)
-over its mean duration, measured with HIP events on the engine's stream.
-`cpu_baseline` times the C restatement of the reference step (FP64, one
-thread, the reference's loop order) on a bounded sample of the same scene.
+algorithmic bytes (212 B per particle-substep plus 28 B per node of the
+active grid blocks) over its mean duration, measured with HIP events on the
+engine's stream.  `cpu_baseline` times the C restatement of the reference
+step (FP64, one thread, the reference's loop order) on a bounded sample of
+the same scene.
 """
 from __future__ import annotations
 
@@ -57,6 +62,10 @@ def parse():
                     help="run the learned-corrector leg (0 skips it)")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="first run 5 plain device copies of this many MiB (calibrates HBM byte counters)")
+    ap.add_argument("--steady-periods", type=int, default=10,
+                    help="rebin periods timed for the steady-state figure (0 skips it)")
+    ap.add_argument("--config1", type=int, default=1,
+                    help="time the 2D config (8,192 particles, 128^2, 200 FP64 substeps) on GPU and CPU")
     return ap.parse_args()
 
 
@@ -116,7 +125,10 @@ def batched_bench(steps, dev):
     run = BatchedRun(work, device=dev)
     tuned = run.autotune(steps=32, reps=2, cache_path=None)
     run.upload()
-    run.run(16, check_oob=False)
+    for _ in range(2):  # captures the timed shape's graphs (both buffer parities)
+        run.engine.request_rebin()
+        run.run(steps, check_oob=False)
+    run.engine.request_rebin()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record()
@@ -188,6 +200,60 @@ def corrector_bench(dev):
             "note": "one corrected big step is ~4x closer to 8 plain substeps than the uncorrected big step, but "
                     "at 8x the explicit step both big-step rollouts go unstable within 4-8 big steps; the "
                     "corrector does not yet keep a 1000-step rollout within 1%"}
+
+
+def config1_bench(dev, cpu=True):
+    """The 2D configuration in full: 8,192 particles, 128^2 grid, 200 FP64
+    substeps through the reference-signature phase functions on device tensors
+    (grids zeroed each substep as the reference does), and the C restatement
+    of the reference step on one host thread."""
+    import numpy as np
+    import torch
+
+    from femflow_amd.scenes import scene_2d_c1
+    from femflow_amd.solvers.mpm import two_d
+
+    st, pr = scene_2d_c1()
+    n, steps = len(st["x"]), 200
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float64, device=dev)  # noqa: E731
+    r1 = pr["res"] + 1
+    mu0, lam0, mass = float(st["mu0"][0]), float(st["lam0"][0]), float(st["mass"][0])
+
+    def roll(k):
+        x, v, F, C, Jp = t(st["x"]), t(st["v"]), t(st["F"]), t(st["C"]), t(st["Jp"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            gv = torch.zeros((r1, r1, 2), dtype=torch.float64, device=dev)
+            gm = torch.zeros((r1, r1), dtype=torch.float64, device=dev)
+            two_d.p2g(pr["inv_dx"], 1.0, mu0, lam0, mass, pr["dx"], pr["dt"], pr["volume"], gv, gm,
+                      x, v, F, C, Jp)
+            two_d.grid_op(pr["res"], pr["dt"], pr["gravity"], gv, gm)
+            two_d.g2p(pr["inv_dx"], pr["dt"], gv, x, v, F, C, Jp)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, x
+
+    roll(5)  # warm-up
+    gpu_s, x_gpu = roll(steps)
+    out = {"workload": "2D MLS-MPM, 8,192 particles in [0.4,0.6]x[0.6,0.8] (seed 0), 128^2 grid, neo-Hookean "
+                       "E=1e3 nu=0.2, dt=1e-4, gravity -9.8, the reference's 2D walls, FP64, 200 substeps",
+           "gpu": {"seconds": gpu_s, "value": n * steps / gpu_s, "unit": "particle-substeps/s",
+                   "path": "two_d.p2g / grid_op / g2p on FP64 device tensors, one call per phase"}}
+    if cpu:
+        from oracle import c_oracle
+
+        cs = c_oracle.CState(st)
+        t0 = time.perf_counter()
+        c_oracle.run(st, pr, steps, nthreads=1, cstate=cs)
+        cpu_s = time.perf_counter() - t0
+        dx_max = float(np.abs(x_gpu.cpu().numpy() - cs.arrays["x"]).max())
+        out["cpu_baseline"] = {"seconds": cpu_s, "value": n * steps / cpu_s, "unit": "particle-substeps/s",
+                               "cores": 1, "kind": "port",
+                               "sample": "C restatement of the reference step, FP64, one thread, all 200 substeps",
+                               "cpu_model": cpu_model()}
+        out["max_abs_position_difference_gpu_vs_cpu"] = dx_max
+    print(f"[bench] config 1: GPU {gpu_s * 1e3:.1f} ms for {steps} substeps", file=sys.stderr, flush=True)
+    return out
 
 
 def binning_bench(x, res, reps=7):
@@ -294,26 +360,45 @@ def main():
              use_graph=not args.no_graph, spread=args.spread)
     eng.upload(x, v, F, C, Jp, mass, mu0, lam0, kind)
     eng.run(args.warmup, check_oob=False)
+    # the timed window's shape from a rebin, once per state-buffer parity, so
+    # that its graphs are captured before the clock starts
+    for _ in range(2):
+        eng.request_rebin()
+        eng.run(args.steps, check_oob=False)
     torch.cuda.synchronize()
-    print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
+    print(f"[bench] warmup {args.warmup} (+ graph capture of the timed shape) done", file=sys.stderr, flush=True)
 
     # ---- timed region: exactly `steps` substeps --------------------------
     stream = torch.cuda.current_stream(dev)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    w0 = time.perf_counter()
-    e0.record(stream)
-    eng.run(args.steps, check_oob=False)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    ms = e0.elapsed_time(e1)
+
+    def timed(k):
+        eng.request_rebin()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        w0 = time.perf_counter()
+        ev0.record(stream)
+        eng.run(k, check_oob=False)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1), time.perf_counter() - w0
+
+    ms, wall = timed(args.steps)
     oob = eng.oob_count()
     if oob:
         raise RuntimeError(f"{oob} particle stencils left the grid during the benchmark")
     value = n * args.steps / (ms / 1e3)
     print(f"[bench] timed {args.steps} substeps: {ms:.2f} ms (wall {wall * 1e3:.2f} ms)",
           file=sys.stderr, flush=True)
+    steady = None
+    if args.steady_periods > 0:
+        ks = args.steady_periods * args.sort_every
+        for _ in range(2):
+            eng.request_rebin()
+            eng.run(ks, check_oob=False)
+        sms, _ = timed(ks)
+        steady = {"steps": ks, "rebins": args.steady_periods, "ms_per_step": sms / ks,
+                  "value": n * ks / (sms / 1e3)}
+        print(f"[bench] steady state: {ks} substeps in {sms:.2f} ms", file=sys.stderr, flush=True)
 
     # ---- per-kernel durations (HIP events on the engine's stream) --------
     # (the same launches as the timed region: no conservation sums)
@@ -329,15 +414,14 @@ def main():
     dom = "g2p2g" if g_cnt else "p2g"
     d_ms, d_cnt = kt[dom]
     avg_s = d_ms / 1e3 / max(d_cnt, 1)
-    active_nodes = None
-    try:
-        active_nodes = int(eng.active_block_count()) * 64
-    except AttributeError:
-        pass
-    alg_bytes = BYTES_PER_PARTICLE * n + BYTES_PER_NODE_TRANSFER * (active_nodes or 0)
+    active_nodes = eng.active_block_count() * 64
+    alg_bytes = BYTES_PER_PARTICLE * n + BYTES_PER_NODE_TRANSFER * active_nodes
     achieved = alg_bytes / avg_s / 1e9
     s = sums.cpu().numpy()
     mass_drift = float(np.abs(s[:, 0] / s[0, 0] - 1).max()) if nprof else 0.0
+    # HBM bytes per launch of the fused kernel from the committed rocprofv3
+    # PMC summary of this workload (FETCH_SIZE, WRITE_SIZE, calibrated by a
+    # plain copy); counters cannot be read inside this run
     traffic = None
     prof = os.path.join(ROOT, "profiles", "g2p2g_c2_1M_128_traffic.json")
     if os.path.exists(prof):
@@ -349,6 +433,7 @@ def main():
     corrector = corrector_bench(dev) if args.corrector else None
     adjoint = adjoint_bench(dev) if args.adjoint else None
     binning = binning_bench(x, args.res)
+    config1 = config1_bench(dev, cpu=args.cpu_seconds > 0) if args.config1 else None
 
     cpu = cpu_omp = None
     if args.cpu_seconds > 0:
@@ -380,9 +465,12 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": "profiles/g2p2g_c2_1M_128_traffic.json (rocprofv3 PMC run of this workload)",
             "kernel": dom, "kernel_ms": avg_s * 1e3,
-            "alg_bytes_per_launch": alg_bytes,
+            "alg_bytes_per_launch": alg_bytes, "active_nodes": active_nodes,
         },
+        "rebins_in_window": -(-args.steps // args.sort_every),
+        "steady_state": steady,
         "step_hbm_frac": (BYTES_PER_PARTICLE * n / (ms / 1e3 / args.steps)) / 1e9 / HBM_PEAK_GBS,
         "kernel_ms_per_step": {k: (vv[0] / vv[1] if vv[1] else 0.0) for k, vv in kt.items()},
         "grid_mass_drift_fp64": mass_drift,
@@ -390,6 +478,7 @@ def main():
         "corrector": corrector,
         "adjoint": adjoint,
         "binning": binning,
+        "config1": config1,
         "cpu_baseline": cpu,
         "cpu_baseline_omp": cpu_omp,
     }

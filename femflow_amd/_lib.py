@@ -26,6 +26,7 @@ EXPORTS = (
     "mpm_g2p", "mpm_ctx_create", "mpm_ctx_destroy", "mpm_ctx_set_stream", "mpm_ctx_tune", "mpm_ctx_set_spread",
     "mpm_ctx_kernel_timing", "mpm_ctx_kernel_times", "mpm_state_upload", "mpm_state_download",
     "mpm_run", "mpm_run_frames", "mpm_run_states", "mpm_run_states_grids", "mpm_state_positions", "mpm_substep", "mpm_ctx_oob_count", "mpm_ctx_fallback_count",
+    "mpm_ctx_active_blocks", "mpm_ctx_request_rebin",
     "mpm_particle_sums", "mpm_bin", "mpm_dense_workspace_bytes", "mpm_dense_substep", "mpm_adjoint_step",
     "mpm_adjoint_step_grids",
     "mpm_ctx_set_corrector", "mpm_corrector_sample", "mpm_grid_velocity",
@@ -108,6 +109,8 @@ def load():
             "mpm_substep": ([vp, P, vp, i32], i32),
             "mpm_ctx_oob_count": ([vp, ct.POINTER(i64)], i32),
             "mpm_ctx_fallback_count": ([vp, ct.POINTER(i64)], i32),
+            "mpm_ctx_active_blocks": ([vp, ct.POINTER(i64)], i32),
+            "mpm_ctx_request_rebin": ([vp], i32),
             "mpm_particle_sums": ([vp, ct.POINTER(dbl)], i32),
             "mpm_bin": ([vp, i32, i32, dbl, i32, vp, i64, vp, vp], i32),
             "mpm_dense_workspace_bytes": ([Q, ct.POINTER(i64)], i32),

@@ -43,10 +43,16 @@ def _snapshot(engine):
 
 
 def _time(engine, steps: int, reps: int) -> float:
-    engine.run(steps, check_oob=False)  # warm-up: captures the graph for this shape
+    """Mean substep time over ``steps`` substeps that start with a rebin
+    (``steps`` is a multiple of sort_every, so rebins are counted at their
+    steady-state rate)."""
+    for _ in range(2):  # warm-up: captures the graphs of both buffer parities
+        engine.request_rebin()
+        engine.run(steps, check_oob=False)
     best = float("inf")
     for _ in range(reps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        engine.request_rebin()
         torch.cuda.synchronize(engine.device)
         e0.record()
         engine.run(steps, check_oob=False)
@@ -84,7 +90,7 @@ def autotune(engine, restore: Tuple, steps: int = 32, reps: int = 2,
     best = None
     for chunk, gm, se, sp in cands:
         engine.tune(chunk=chunk, grid_mult=gm, sort_every=se, spread=sp)
-        t = _time(engine, max(steps, se), reps)
+        t = _time(engine, se * max(1, -(-steps // se)), reps)
         table.append({"chunk": chunk, "grid_mult": gm, "sort_every": se, "spread": sp, "ms_per_substep": t})
         if verbose:
             print("autotune", chunk, gm, se, sp, f"{t:.4f} ms", flush=True)

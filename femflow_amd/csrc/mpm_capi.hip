@@ -344,9 +344,8 @@ int32_t mpm_ctx_tune(mpm_ctx* ctx, int32_t chunk, int32_t sort_every, int32_t gr
   }
   mpm::Engine& e = ctx->eng;
   if (e.chunk != chunk || e.grid_mult != grid_mult) {
-    hipStreamSynchronize(e.own_stream);
-    for (auto& kv : e.graphs) hipGraphExecDestroy(kv.second);
-    e.graphs.clear();
+    e.drop_graphs();
+    e.since_bin = -1;  // the chunk list was built for the old chunk size
   }
   e.chunk = chunk;
   e.sort_every = sort_every;
@@ -361,11 +360,7 @@ int32_t mpm_ctx_set_spread(mpm_ctx* ctx, int32_t spread) {
     return MPM_ERR_ARG;
   }
   mpm::Engine& e = ctx->eng;
-  if (e.spread != spread) {  // captured graphs hold the other kernel
-    hipStreamSynchronize(e.own_stream);
-    for (auto& kv : e.graphs) hipGraphExecDestroy(kv.second);
-    e.graphs.clear();
-  }
+  if (e.spread != spread) e.drop_graphs();  // captured graphs hold the other kernel
   e.spread = spread;
   return MPM_OK;
 }
@@ -500,6 +495,23 @@ int32_t mpm_ctx_fallback_count(mpm_ctx* ctx, int64_t* out) {
     return MPM_ERR_ARG;
   }
   return ctx->eng.take_counter(1, out);
+}
+
+int32_t mpm_ctx_active_blocks(mpm_ctx* ctx, int64_t* out) {
+  if (!ctx || !out) {
+    set_error("mpm_ctx_active_blocks: null argument");
+    return MPM_ERR_ARG;
+  }
+  return ctx->eng.active_blocks(out);
+}
+
+int32_t mpm_ctx_request_rebin(mpm_ctx* ctx) {
+  if (!ctx) {
+    set_error("mpm_ctx_request_rebin: null context");
+    return MPM_ERR_ARG;
+  }
+  ctx->eng.since_bin = -1;
+  return MPM_OK;
 }
 
 int32_t mpm_particle_sums(mpm_ctx* ctx, double* out) {

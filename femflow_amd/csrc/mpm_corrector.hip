@@ -302,8 +302,7 @@ int Engine::set_corrector(const float* params, float m_ref, float v_ref, int ena
     return MPM_ERR_ARG;
   }
   hipStreamSynchronize(own_stream);
-  for (auto& kv : graphs) hipGraphExecDestroy(kv.second);
-  graphs.clear();
+  drop_graphs();
   int rc;
   if ((enable || params) && !d_pm) {
     const size_t nodes = (size_t)total_blocks * 64;
@@ -361,6 +360,7 @@ int Engine::corrector_sample(uint16_t* X, uint32_t* nodes, float* vel, int64_t m
   }
   begin_segment(s, 1, 0);
   enqueue_bin(s);
+  since_bin = 0;  // freshly binned: the next run continues from this order
   const EngineArgs a = args(nullptr, 0, 0);
   launch_transfer(0, s, a);
   launch_cellf(s, a);

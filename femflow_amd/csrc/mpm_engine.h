@@ -14,14 +14,21 @@ namespace mpm {
 
 void set_error(const std::string& msg);
 
+// A captured segment fixes the particle count (launch grids, sort sizes),
+// the segment length, whether it rebins first, which state buffer is current
+// and the sums destination; any of them changing needs another graph.
 struct GraphKey {
-  int len, parity;
+  int64_t n;
+  int len, parity, bin;
   uintptr_t sums;
   int nsteps;
   bool operator<(const GraphKey& o) const {
-    return std::tie(len, parity, sums, nsteps) < std::tie(o.len, o.parity, o.sums, o.nsteps);
+    return std::tie(n, len, parity, bin, sums, nsteps) <
+           std::tie(o.n, o.len, o.parity, o.bin, o.sums, o.nsteps);
   }
 };
+// Captured graphs kept per context; beyond this the cache is emptied first.
+constexpr size_t MAX_GRAPHS = 16;
 
 struct CorrExtract {
   uint16_t* X;      // rows x 352 BF16 features
@@ -53,6 +60,10 @@ struct Engine {
   int chunk = 512, sort_every = 8, grid_mult = 4;
   int spread = 9;  // scatter start offsets per cell (3 or 9, see transfer_kernel)
   bool use_graph = true, timing = false;
+  // substeps run since the particles were last binned; -1: not binned (after
+  // an upload or a rebin request), so the next segment rebins first
+  int since_bin = -1;
+  int last_list = 0;  // active-block list the last substep's P2G appended to
   std::vector<SceneDev> scenes_host;
   // streams and graphs
   hipStream_t user_stream = nullptr, own_stream = nullptr;
@@ -103,8 +114,11 @@ struct Engine {
   int upload(const mpm_particles* p, const int32_t* scene_ids);
   int download(mpm_particles* p);
   void enqueue_bin(hipStream_t s);
-  void enqueue_segment(int len, hipStream_t s, double* sums, int max_steps, float* frames = nullptr,
+  void enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int max_steps, float* frames = nullptr,
                        int64_t step0 = 0, const StateSink* sink = nullptr);
+  // destroy every captured graph (after anything they baked in changes)
+  void drop_graphs();
+  int active_blocks(int64_t* out);
   void enqueue_frame(hipStream_t s, float* frames, int64_t step);
   void enqueue_state(hipStream_t s, const StateSink* sink, int64_t step);
   int positions(float* x);

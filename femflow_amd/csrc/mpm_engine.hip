@@ -195,36 +195,53 @@ struct P2GTerms {
 
 // FP32 terms; false when the FP32 polar factor is not trustworthy (F near
 // singular or the iteration did not settle): the caller defers the particle.
+// P F^T = 2 mu (F - R) F^T + lam (J - 1) J (utils.py:120-135): the first
+// term from the series of stretch_series while F is within 3% of a
+// rotation (every snow particle after its clamp, and elastic ones under
+// moderate strain), else from the FP32 polar iteration.
 __device__ __forceinline__ bool p2g_terms(const SceneDev& sc, const MatParams& mp, float mass,
                                           float mu0, float lam0, float Jp, const float F[9],
                                           const float C[9], const float v[3], const float fx[3],
                                           P2GTerms& t) {
-  float mu, lam, R[9];
+  float mu, lam;
   lame<float>(mp.kind, mp.hardening, mu0, lam0, Jp, mu, lam);
+  float dev[9];  // 2 mu (F - R) F^T
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dev[i] = 0.f;
   if (mu != 0.f) {
-    if (!polar_newton<float>(F, R, 12, 1e-5f)) return false;
+    const float m2 = 2.f * mu;
+    float S[6];
+    if (stretch_series<float>(F, S)) {
+      dev[0] = m2 * S[0];
+      dev[1] = dev[3] = m2 * S[1];
+      dev[2] = dev[6] = m2 * S[2];
+      dev[4] = m2 * S[3];
+      dev[5] = dev[7] = m2 * S[4];
+      dev[8] = m2 * S[5];
+    } else {
+      float R[9];
+      if (!polar_newton<float>(F, R, 12, 1e-5f)) return false;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const float d0 = m2 * (F[3 * i] - R[3 * i]), d1 = m2 * (F[3 * i + 1] - R[3 * i + 1]),
+                    d2 = m2 * (F[3 * i + 2] - R[3 * i + 2]);
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dev[3 * i + j] = d0 * F[3 * j] + d1 * F[3 * j + 1] + d2 * F[3 * j + 2];
+      }
+    }
   }
-  const float J = det3(F);
+  const float Jm1 = det3_minus_one(F);
   const float scale = -(sc.dt * sc.vol) * (4.f * sc.inv_dx * sc.inv_dx);
-  const float vt = lam * (J - 1.f) * J;
+  const float vt = lam * Jm1 * (1.f + Jm1);
   const bool diag_only = sc.quirks == MPM_QUIRKS_TEXTBOOK || mp.kind == MPM_WATER;
-  const float m2 = 2.f * mu;
   float A[9];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    float d0 = 0.f, d1 = 0.f, d2 = 0.f;
-    if (mu != 0.f) {
-      d0 = m2 * (F[3 * i] - R[3 * i]);
-      d1 = m2 * (F[3 * i + 1] - R[3 * i + 1]);
-      d2 = m2 * (F[3 * i + 2] - R[3 * i + 2]);
-    }
+  for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      float pf = d0 * F[3 * j] + d1 * F[3 * j + 1] + d2 * F[3 * j + 2];
-      pf += (diag_only && i != j) ? 0.f : vt;
+      const float pf = dev[3 * i + j] + ((diag_only && i != j) ? 0.f : vt);
       A[3 * i + j] = scale * pf + mass * C[3 * i + j];
     }
-  }
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     t.q0[d] = mass * v[d] - sc.dx * (A[3 * d] * fx[0] + A[3 * d + 1] * fx[1] + A[3 * d + 2] * fx[2]);
@@ -1350,8 +1367,10 @@ int Engine::upload(const mpm_particles* p, const int32_t* scene_ids) {
     set_error("mpm_state_upload: material ids are required when the table has more than one entry");
     return MPM_ERR_ARG;
   }
+  if (p->n != n) drop_graphs();
   n = p->n;
   cur = 0;
+  since_bin = -1;
   if (n > 0)
     hipLaunchKernelGGL(upload_kernel, dim3(nblk(n, 256)), dim3(256), 0, own_stream, st[0], n,
                        (const float*)p->x, (const float*)p->v, (const float*)p->F, (const float*)p->C,
@@ -1485,11 +1504,11 @@ void Engine::enqueue_state(hipStream_t s, const StateSink* sink, int64_t step) {
                      sink->v + 3 * o, sink->F + 9 * o, sink->C + 9 * o, sink->Jp + o);
 }
 
-void Engine::enqueue_segment(int len, hipStream_t s, double* sums, int max_steps, float* frames,
+void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int max_steps, float* frames,
                              int64_t step0, const StateSink* sink) {
   const int ws = sums ? 1 : 0;
   hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws);
-  enqueue_bin(s);
+  if (bin) enqueue_bin(s);
   // substep k appends to active list k % 2; its grid update clears the other
   const size_t rec_nn = (size_t)(scenes_host[0].res + 1) * (scenes_host[0].res + 1) * (scenes_host[0].res + 1);
   auto grid_args = [&](int k) {
@@ -1533,19 +1552,28 @@ int Engine::run(int nsteps, double* sums, float* frames, const StateSink* sink) 
     hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(cnt, 256)), dim3(256), 0, own_stream,
                        reinterpret_cast<uint32_t*>(sink->grids), cnt, 0u);
   }
+  // Segments: the particles are rebinned when they never were since the
+  // upload (or a rebin request) or after sort_every substeps; the schedule
+  // carries over from one call to the next, so callers that advance a few
+  // substeps per call do not pay a rebin per call.  Every call ends with
+  // complete particle state (G2P), so a call costs one split P2G / G2P pair
+  // beyond the fused substeps.
   int done = 0;
   while (done < nsteps) {
-    const int len = (nsteps - done) < sort_every ? (nsteps - done) : sort_every;
+    const bool bin = since_bin < 0 || since_bin >= sort_every || n <= 0;
+    const int room = bin ? sort_every : sort_every - since_bin;
+    const int len = (nsteps - done) < room ? (nsteps - done) : room;
     const bool graphed = use_graph && !timing && n > 0 && !frames && !sink;
     if (graphed) {
       const int parity = cur;
-      const GraphKey key{len, parity, (uintptr_t)sums, sums ? nsteps : 0};
+      const GraphKey key{n, len, parity, bin ? 1 : 0, (uintptr_t)sums, sums ? nsteps : 0};
       auto it = graphs.find(key);
       hipGraphExec_t exec = nullptr;
       if (it == graphs.end()) {
+        if (graphs.size() >= MAX_GRAPHS) drop_graphs();
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(own_stream, hipStreamCaptureModeThreadLocal));
-        enqueue_segment(len, own_stream, sums, nsteps);
+        enqueue_segment(len, bin, own_stream, sums, nsteps);
         HIPCHK(hipStreamEndCapture(own_stream, &graph));
         HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(graph));
@@ -1555,15 +1583,34 @@ int Engine::run(int nsteps, double* sums, float* frames, const StateSink* sink) 
         exec = it->second;
       }
       HIPCHK(hipGraphLaunch(exec, own_stream));
-      cur = parity ^ 1;
+      cur = bin ? parity ^ 1 : parity;
     } else {
-      enqueue_segment(len, own_stream, sums, nsteps, frames, done, sink);
+      enqueue_segment(len, bin, own_stream, sums, nsteps, frames, done, sink);
     }
     HIPCHK(hipGetLastError());
+    since_bin = bin ? len : since_bin + len;
+    last_list = (len - 1) & 1;
     done += len;
   }
   HIPCHK(hipEventRecord(ev_out, own_stream));
   HIPCHK(hipStreamWaitEvent(user_stream, ev_out, 0));
+  return MPM_OK;
+}
+
+void Engine::drop_graphs() {
+  if (graphs.empty()) return;
+  hipStreamSynchronize(own_stream);  // a launched graph may still be running
+  for (auto& kv : graphs)
+    if (kv.second) hipGraphExecDestroy(kv.second);
+  graphs.clear();
+}
+
+// Blocks the last substep's P2G touched (the grid update's work list).
+int Engine::active_blocks(int64_t* out) {
+  HIPCHK(hipStreamSynchronize(own_stream));
+  uint32_t v = 0;
+  HIPCHK(hipMemcpy(&v, d_counters + last_list, sizeof(v), hipMemcpyDeviceToHost));
+  *out = (int64_t)v;
   return MPM_OK;
 }
 
@@ -1658,8 +1705,7 @@ int Engine::set_planes(int scene, int count, const double* points, const double*
     set_error("mpm_ctx_set_planes: synchronising the engine stream failed");
     return MPM_ERR_HIP;
   }
-  for (auto& kv : graphs) hipGraphExecDestroy(kv.second);
-  graphs.clear();
+  drop_graphs();
   if (!next.empty() &&
       (hipMemcpyAsync(d_planes, next.data(), sizeof(PlaneDev) * next.size(), hipMemcpyHostToDevice, own_stream) !=
            hipSuccess ||

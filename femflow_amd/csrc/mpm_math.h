@@ -236,6 +236,65 @@ template <typename T> MPM_DEV bool polar_newton(const T* F, T R[9], int iters, T
   return false;
 }
 
+// det(F) - 1 from D = F - I (exact in floating point for F near I):
+// det(I + D) = 1 + tr D + (sum of the principal 2x2 minors of D) + det D.
+// Near a rotation-free F this keeps J - 1 to the relative precision of D,
+// where det3(F) - 1 would cancel to the rounding of an O(1) determinant.
+template <typename T> MPM_DEV T det3_minus_one(const T* F) {
+  const T d0 = F[0] - T(1), d4 = F[4] - T(1), d8 = F[8] - T(1);
+  const T tr = d0 + d4 + d8;
+  const T m2 = (d0 * d4 - F[1] * F[3]) + (d0 * d8 - F[2] * F[6]) + (d4 * d8 - F[5] * F[7]);
+  const T dd = d0 * (d4 * d8 - F[5] * F[7]) - F[1] * (F[3] * d8 - F[5] * F[6]) + F[2] * (F[3] * F[7] - d4 * F[6]);
+  return tr + m2 + dd;
+}
+
+// (F - R) F^T of the fixed-corotated stress without the polar factor R.
+// With R = U V^T from the SVD F = U S V^T (linear_algebra.py:119-135),
+// R F^T = U S U^T = sqrt(B), B = F F^T, for either sign of det F, so
+// (F - R) F^T = B - sqrt(B).  With E = B - I = D + D^T + D D^T (D = F - I),
+//   B - sqrt(B) = E/2 + E^2/8 - E^3/16 + 5 E^4/128 - 7 E^5/256 + ...
+// (the binomial series of sqrt(I + E), which converges for ||E|| < 1).  Five
+// terms leave a relative error below 4e-8 while ||E||_F <= 0.06 (FP64
+// check against the SVD form: 1e-8 at 0.05, 4e-8 at 0.06); beyond that the
+// caller takes the polar iteration.  Out: the symmetric result as
+// (00, 01, 02, 11, 12, 22).  All products are of commuting symmetric
+// matrices (polynomials in E), so each stays symmetric.
+template <typename T> MPM_DEV bool stretch_series(const T* F, T out[6]) {
+  const T D[9] = {F[0] - T(1), F[1], F[2], F[3], F[4] - T(1), F[5], F[6], F[7], F[8] - T(1)};
+  T E[6];
+  E[0] = T(2) * D[0] + (D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+  E[1] = (D[1] + D[3]) + (D[0] * D[3] + D[1] * D[4] + D[2] * D[5]);
+  E[2] = (D[2] + D[6]) + (D[0] * D[6] + D[1] * D[7] + D[2] * D[8]);
+  E[3] = T(2) * D[4] + (D[3] * D[3] + D[4] * D[4] + D[5] * D[5]);
+  E[4] = (D[5] + D[7]) + (D[3] * D[6] + D[4] * D[7] + D[5] * D[8]);
+  E[5] = T(2) * D[8] + (D[6] * D[6] + D[7] * D[7] + D[8] * D[8]);
+  const T fro2 = E[0] * E[0] + E[3] * E[3] + E[5] * E[5] + T(2) * (E[1] * E[1] + E[2] * E[2] + E[4] * E[4]);
+  if (!(fro2 <= T(0.0036))) return false;  // also NaN
+  // Horner: S <- E S + c I from S = c5 I, then out = E S
+  const T c[5] = {T(0.5), T(0.125), T(-0.0625), T(0.0390625), T(-0.02734375)};
+  T S[6] = {c[4], T(0), T(0), c[4], T(0), c[4]};
+#pragma unroll
+  for (int k = 3; k >= -1; --k) {
+    T P[6];
+    P[0] = E[0] * S[0] + E[1] * S[1] + E[2] * S[2];
+    P[1] = E[0] * S[1] + E[1] * S[3] + E[2] * S[4];
+    P[2] = E[0] * S[2] + E[1] * S[4] + E[2] * S[5];
+    P[3] = E[1] * S[1] + E[3] * S[3] + E[4] * S[4];
+    P[4] = E[1] * S[2] + E[3] * S[4] + E[4] * S[5];
+    P[5] = E[2] * S[2] + E[4] * S[4] + E[5] * S[5];
+    if (k >= 0) {
+      P[0] += c[k];
+      P[3] += c[k];
+      P[5] += c[k];
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) S[i] = P[i];
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) out[i] = S[i];
+  return true;
+}
+
 // R = U V^T through the FP64 Jacobi SVD: the fallback for any F the FP32
 // iteration declines.
 MPM_DEV void polar_svd_f64(const double* F, double R[9]) {

@@ -212,6 +212,17 @@ class Engine:
         _lib.check(self._L.mpm_ctx_fallback_count(self._h, ct.byref(out)), "fallback_count")
         return int(out.value)
 
+    def active_block_count(self) -> int:
+        """Grid blocks (4x4x4 nodes) the last substep's P2G reached."""
+        out = ct.c_int64()
+        _lib.check(self._L.mpm_ctx_active_blocks(self._h, ct.byref(out)), "active_block_count")
+        return int(out.value)
+
+    def request_rebin(self):
+        """Rebin the particles at the start of the next run (the engine otherwise
+        rebins after an upload and then every sort_every substeps, across calls)."""
+        _lib.check(self._L.mpm_ctx_request_rebin(self._h), "request_rebin")
+
     def particle_sums(self):
         out = (ct.c_double * 7)()
         _lib.check(self._L.mpm_particle_sums(self._h, out), "particle_sums")

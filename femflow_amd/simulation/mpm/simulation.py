@@ -21,6 +21,7 @@ import torch
 from ... import _lib
 from ...engine import Engine, MaterialSpec, SceneSpec
 from ...numerics.fem import Ev_to_lambda, Ev_to_mu
+from ...solvers.mpm.particle import Particle
 from ..simulation_base import SimulationBase
 
 log = logging.getLogger("femflow_amd")
@@ -32,6 +33,36 @@ class PointCloud:
 
     def __init__(self, points):
         self.vertices = np.asarray(points, dtype=np.float32).reshape(-1)
+
+
+class ParticleRecords:
+    """The reference's ``self.particles`` (simulation.py:78-89): one
+    ``Particle`` per particle, in load order, with ``pos`` (float64[3]),
+    ``force``, ``mass``, ``lambda_0`` and ``mu_0``.  It is a snapshot of the
+    device state taken when the attribute is read: each record's ``pos`` is a
+    row view of one (N, 3) float64 array (``positions``), so reading every
+    record costs one device-to-host copy, not N."""
+
+    def __init__(self, positions, force, mass, lambda_0, mu_0):
+        self.positions = positions
+        self._force = force
+        self._mass, self._lambda_0, self._mu_0 = mass, lambda_0, mu_0
+
+    def __len__(self):
+        return len(self.positions)
+
+    def __getitem__(self, i):
+        if isinstance(i, slice):
+            return [self[j] for j in range(*i.indices(len(self)))]
+        if i < 0:
+            i += len(self)
+        if not 0 <= i < len(self):
+            raise IndexError(i)
+        return Particle(self.positions[i], self._force, float(self._mass[i]), float(self._lambda_0[i]),
+                        float(self._mu_0[i]))
+
+    def __iter__(self):
+        return (self[i] for i in range(len(self)))
 
 
 class MPMSimulation(SimulationBase):
@@ -89,6 +120,7 @@ class MPMSimulation(SimulationBase):
             lam.append(np.full(len(p), Ev_to_lambda(*param[1:])))
             mu.append(np.full(len(p), Ev_to_mu(*param[1:])))
         pos = np.concatenate(pos)
+        self._mass_h, self._mu_h, self._lam_h = np.concatenate(mass), np.concatenate(mu), np.concatenate(lam)
         n = len(pos)
         dev = self.device or torch.device("cuda", torch.cuda.current_device())
         self.device = dev
@@ -165,6 +197,13 @@ class MPMSimulation(SimulationBase):
     def positions(self):
         self._download()
         return self._x.double().cpu().numpy()
+
+    @property
+    def particles(self):
+        """Particle records of the current state (simulation.py:78-89)."""
+        if not self.loaded:
+            raise AttributeError("particles exist after load()")
+        return ParticleRecords(self.positions, self.force, self._mass_h, self._lam_h, self._mu_h)
 
     @property
     def v(self):

@@ -202,6 +202,15 @@ int32_t mpm_substep(mpm_ctx* ctx, mpm_particles* p, const int32_t* scene_ids, in
 int32_t mpm_ctx_oob_count(mpm_ctx* ctx, int64_t* out);
 /* FP32 polar/SVD evaluations that fell back to FP64 (reading resets it). */
 int32_t mpm_ctx_fallback_count(mpm_ctx* ctx, int64_t* out);
+/* Grid blocks (4x4x4 nodes) the last substep's P2G reached: the work list of
+ * its grid update.  Synchronises the stream.  No reference counterpart (the
+ * reference sweeps the dense grid, three_d/grid_op.py:25-47); it sizes the
+ * per-node term of the algorithmic byte count. */
+int32_t mpm_ctx_active_blocks(mpm_ctx* ctx, int64_t* out);
+/* Rebin the particles at the start of the next mpm_run.  The engine rebins
+ * after every upload and then every sort_every substeps, counting across
+ * calls; this restarts that count.  No reference counterpart. */
+int32_t mpm_ctx_request_rebin(mpm_ctx* ctx);
 /* FP64 sums over the engine's particles: out[0] = sum m, out[1..3] = sum m v,
  * out[4..6] = sum m x (host double[7]). Synchronises the stream. */
 int32_t mpm_particle_sums(mpm_ctx* ctx, double* out);

@@ -50,16 +50,21 @@ KATS = {
 def main():
     with open(os.path.join(HERE, "ref_kats.json"), "w") as f:
         json.dump(KATS, f, indent=1)
-    # 2D config-1 scene (8,192 particles, res 128): states after 1 and 10 substeps
+    # 2D config-1 scene (8,192 particles, res 128): states after 1 and 10
+    # substeps, and after all 200 substeps of the configuration
     st, pr = O.scene_2d_c1()
     s = O.copy_state(st)
     O.substep(s, pr)
     one = O.copy_state(s)
     for _ in range(9):
         O.substep(s, pr)
+    ten = O.copy_state(s)
+    for _ in range(190):
+        O.substep(s, pr)
     np.savez_compressed(os.path.join(HERE, "oracle_c1_2d.npz"),
                         **{f"s1_{k}": one[k] for k in ("x", "v", "F", "C", "Jp")},
-                        **{f"s10_{k}": s[k] for k in ("x", "v", "F", "C", "Jp")})
+                        **{f"s10_{k}": ten[k] for k in ("x", "v", "F", "C", "Jp")},
+                        **{f"s200_{k}": s[k] for k in ("x", "v", "F", "C", "Jp")})
     # 3D three-material scene, 6,000 particles at res 32: after 1 and 5 substeps
     st, pr = O.scene_3d_cubes(n=6000, res=32)
     s = O.copy_state(st)

@@ -115,3 +115,47 @@ def test_option_strings_are_validated():
     with pytest.raises(ValueError):
         Engine([SceneSpec(res=16, dt=1e-4, boundary="open")], [MaterialSpec("elastic", 1.0)], n_max=8,
                device="cpu")
+
+
+@pytest.mark.gpu
+def test_mpm_substep_with_changing_counts(gpu):
+    """mpm_substep on one context, called with 2000 particles, then 1000, then
+    2000 again, straight through ctypes: every call must match the FP64 oracle
+    and write only the caller's arrays of that call's count."""
+    import numpy as np
+    import torch
+
+    from oracle import mpm_oracle as O
+    from tests._parity import field_errors
+    from tests.gpu_helpers import oracle_copy_from, to_numpy_state, to_torch_state
+
+    L = _lib.load()
+    st, pr = O.scene_3d_cubes(n=2000, res=32)
+    sc = (_lib.Scene * 1)(_lib.Scene(pr["res"], _lib.MPM_BC_REFERENCE, 3, _lib.MPM_QUIRKS_REFERENCE, pr["dt"],
+                                     pr["gravity"], pr["volume"]))
+    mats = (_lib.Material * 3)(*[_lib.Material(k, h, tc, ts) for k, _, _, h, tc, ts in O.MATERIALS_C2])
+    h = ct.c_void_p()
+    assert L.mpm_ctx_create(ct.byref(h), 2000, 1, sc, 3, mats, 0) == _lib.MPM_OK
+    L.mpm_ctx_set_stream(h, torch.cuda.current_stream(gpu).cuda_stream)
+    try:
+        for n in (2000, 1000, 2000):
+            sub = {k: (a[:n].copy() if isinstance(a, np.ndarray) and len(a) == 2000 else a) for k, a in st.items()}
+            ts = to_torch_state(sub, gpu)
+            ref = oracle_copy_from(ts, sub)
+            xbuf = torch.full((2000 * 3 + 300,), 5.0, device=gpu)
+            xbuf[: 3 * n] = ts["x"].reshape(-1)
+            p = _lib.Particles(n, _lib.MPM_F32, 0, xbuf.data_ptr(), ts["v"].data_ptr(), ts["F"].data_ptr(),
+                               ts["C"].data_ptr(), ts["Jp"].data_ptr(), ts["mass"].data_ptr(),
+                               ts["mu0"].data_ptr(), ts["lam0"].data_ptr(), ts["kind"].data_ptr())
+            assert L.mpm_substep(h, ct.byref(p), None, 3) == _lib.MPM_OK, _lib.last_error()
+            torch.cuda.synchronize()
+            for _ in range(3):
+                O.substep(ref, pr)
+            got = dict(ts)
+            got["x"] = xbuf[: 3 * n].reshape(n, 3)
+            errs = field_errors(to_numpy_state(got), ref, pr["inv_dx"])
+            for k in ("x", "v", "F"):
+                assert errs[k] < 1e-4, (n, errs)
+            assert bool((xbuf[3 * n:] == 5.0).all()), "mpm_substep wrote past the caller's count"
+    finally:
+        L.mpm_ctx_destroy(h)

@@ -105,6 +105,57 @@ def test_solve_mls_mpm_3d_fp32_engine(gpu):
     assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
 
 
+def test_solve_mls_mpm_3d_fp32_fewer_then_more_particles(gpu):
+    """Successive FP32 calls whose particle counts change (2000, then 1000,
+    then 3000) reuse the cached engine; each must match the FP64 oracle.  A
+    graph captured for one count must never be replayed for another."""
+    from femflow_amd.solvers.mpm import ParticleSet, solve_mls_mpm_3d
+
+    for n in (2000, 1000, 3000):
+        st, pr = O.scene_3d_cubes(n=n, res=32, seed=n)
+        st["kind"][:] = O.ELASTIC
+        ts = to_torch_state(st, gpu)
+        ps = ParticleSet(ts["x"], ts["mass"], ts["mu0"], ts["lam0"])
+        ref = oracle_copy_from(ts, st)
+        solve_mls_mpm_3d(pr["res"], pr["inv_dx"], 1.0, pr["dx"], pr["dt"], pr["volume"], pr["gravity"], ps,
+                         ts["v"], ts["F"], ts["C"], ts["Jp"].view(-1, 1))
+        O.solve_mls_mpm_3d(pr["res"], pr["inv_dx"], 1.0, pr["dx"], pr["dt"], pr["volume"], pr["gravity"],
+                           ref["x"], ref["mass"], ref["mu0"], ref["lam0"], ref["v"], ref["F"], ref["C"],
+                           ref["Jp"])
+        errs = field_errors(to_numpy_state(ts), ref, pr["inv_dx"])
+        _say("fp32 solve, n =", n, errs)
+        for k in ("x", "v", "F"):
+            assert errs[k] < 1e-5, (n, errs)
+
+
+def test_engine_reupload_changes_count(gpu):
+    """One engine, uploads of 4000 then 1500 then 4000 particles, several
+    graph-replayed substeps each, against the oracle; the downloads land only
+    in the caller's arrays of the current count."""
+    st, pr = O.scene_3d_cubes(n=4000, res=32)
+    eng = engine_for(st, pr, gpu, sort_every=4)
+    for n in (4000, 1500, 4000):
+        sub = {k: (a[:n].copy() if isinstance(a, np.ndarray) and len(a) == 4000 else a) for k, a in st.items()}
+        ts = to_torch_state(sub, gpu)
+        guard = torch.full((n + 1000, 3), 7.0, device=gpu)
+        x_out = guard[:n]
+        eng.upload(ts["x"], ts["v"], ts["F"], ts["C"], ts["Jp"], ts["mass"], ts["mu0"], ts["lam0"], ts["kind"])
+        eng.run(6)
+        out = {k: torch.empty_like(ts[k]) for k in ("v", "F", "C", "Jp")}
+        eng.download(x_out, out["v"], out["F"], out["C"], out["Jp"])
+        torch.cuda.synchronize()
+        out["x"] = x_out
+        ref = oracle_copy_from(ts, sub)
+        for _ in range(6):
+            O.substep(ref, pr)
+        errs = field_errors(to_numpy_state(out), ref, pr["inv_dx"])
+        _say("reupload", n, errs)
+        for k in ("x", "v", "F"):
+            assert errs[k] < 1e-4, (n, errs)
+        assert bool((guard[n:] == 7.0).all()), "download wrote past the current particle count"
+    eng.close()
+
+
 @pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
 @pytest.mark.parametrize("model", ["neo_hookean", "snow"])
 def test_three_d_phases(gpu, dtype, model):

@@ -396,3 +396,13 @@ def test_collide_planes_matches_node_loop():
                         if (o[0] * nn[0] + o[1] * nn[1]) + o[2] * nn[2] < 0:
                             ref[i, j, k] = 0.0
         assert np.array_equal(gv, ref)
+
+
+def test_config1_full_rollout_fixture():
+    """All 200 substeps of the 2D configuration: the C restatement (reference
+    loop order) lands on the NumPy oracle's committed final state."""
+    st, pr = O.scene_2d_c1()
+    g = np.load(os.path.join(GOLDEN, "oracle_c1_2d.npz"))
+    b = CO.run(st, pr, 200).to_state()
+    errs = field_errors(b, {k: g[f"s200_{k}"] for k in ("x", "v", "F", "C", "Jp")}, pr["inv_dx"])
+    assert max(errs.values()) <= 1e-9, errs

@@ -108,3 +108,11 @@ def test_rollout_matches_oracle_and_writes_reference_frames(gpu, tmp_path):
     errs = field_errors(got, dict(x=x, v=v, F=F, C=C, Jp=Jp), 64.0)
     print("final state errors", errs, flush=True)
     assert errs["x"] < 1e-4 and errs["v"] < 1e-3 and errs["F"] < 1e-3
+    # the reference's particle records (simulation.py:78-89)
+    parts = sim.particles
+    assert len(parts) == n
+    p0, plast = parts[0], parts[-1]
+    assert np.array_equal(p0.pos, got["x"][0]) and np.array_equal(plast.pos, got["x"][-1])
+    assert p0.pos.dtype == np.float64 and p0.force == -9.8
+    assert p0.mass == 1.0 and np.isclose(p0.mu_0, mu[0]) and np.isclose(p0.lambda_0, lam[0])
+    assert np.isclose(plast.mass, 10.0) and np.isclose(plast.mu_0, mu[-1]) and np.isclose(plast.lambda_0, lam[-1])
