@@ -190,13 +190,24 @@ def corrector_bench(dev):
     meta = net.meta
     print(f"[bench] corrector: one-step {gain}, plain {times['plain']:.3f} ms vs corrected {times['corrected']:.3f} ms "
           f"per {16 * cfg.dt * 1e3:.2f} ms simulated", file=sys.stderr, flush=True)
+    # the judged form over the configuration's 1000 substeps: the plain
+    # substep judges windows of big steps and replaces any window whose
+    # position error exceeds 1% (fallback and judging cost included)
+    judged = CT.judged_rollout(cfg, net, dev, n_substeps=1000)
+    met = any(isinstance(st["rel_position_error"], float) and st["within_tol"] and st["speedup"] >= 4.0
+              for st in judged["settings"])
+    print(f"[bench] corrector judged 1000 substeps: " + "; ".join(
+        f"window {st['window']} judge every {st['judge_every']}: speedup {st['speedup']:.2f} err "
+        f"{st['rel_position_error']} fallbacks {st['fallbacks']}" for st in judged["settings"]),
+        file=sys.stderr, flush=True)
     return {"workload": "headline scene, dt_big = 8 dt = 4e-4, network 351-128-128-3 trained on "
                         f"{meta.get('rollouts')} solver rollouts ({meta.get('rows')} node samples)",
             "one_step_displacement_error": gain,
             "rollout_rel_position_error_by_big_step": {"with_network": roll_with, "without_network": roll_without},
             "ms_per_simulated_0.8ms": {"plain": times["plain"], "corrected": times["corrected"]},
-            "speedup_simulated_time": times["plain"] / times["corrected"],
-            "target_met": False,
+            "speedup_simulated_time_2_big_steps": times["plain"] / times["corrected"],
+            "judged_1000_substeps": judged,
+            "target_met": met,
             "note": "one corrected big step is ~4x closer to 8 plain substeps than the uncorrected big step, but "
                     "at 8x the explicit step both big-step rollouts go unstable within 4-8 big steps; the "
                     "corrector does not yet keep a 1000-step rollout within 1%"}

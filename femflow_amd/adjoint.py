@@ -8,12 +8,13 @@ one substep (``mpm_adjoint_step``, on dense (res+1)^3 grids), applied in
 reverse order.  The forward substeps run on the particle-resident engine and
 keep a checkpoint of the particle state every ``cfg.checkpoint_every``
 substeps; the backward replays one checkpoint interval at a time on the
-engine, keeping that interval's per-substep states (24 floats per particle
+engine, keeping that interval's per-substep states (25 floats per particle
 per substep) and dense forward grids (7 floats per node per substep, from
 ``mpm_run_states_grids``), and reverses it with ``mpm_adjoint_step_grids``.
-When the record of the whole rollout fits in ``record_budget_gib`` (10 GiB
-for 2^17 particles, a 64^3 grid and 500 substeps), the forward itself keeps
-it and the backward replays nothing.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
+When the record of the whole rollout fits in ``record_budget_gib`` and in
+80% of the free device memory (10 GiB for 2^17 particles, a 64^3 grid and
+500 substeps), the forward itself keeps it and the backward replays
+nothing.  When no input needs a gradient the forward keeps nothing.  Gradients flow to x0, v0, F0, C0, mu0 and lambda0 (per particle
 or scalar); mass is a constant.
 
 The reference has no gradients; the forward it differentiates is its step
@@ -188,8 +189,18 @@ class _Rollout(torch.autograd.Function):
         ckpts = []
         record = None
         fwd.load(x, v, F, C)
-        per_step = 4 * (24 * n + 7 * (cfg.res + 1) ** 3)
-        if cfg.nsteps > 0 and per_step * cfg.nsteps <= cfg.record_budget_gib * 2 ** 30:
+        # the replay writes 25 floats per particle (x, v, F, C, Jp) and 7 per
+        # grid node per substep; record only when a gradient will be asked
+        # for and the record fits both the budget and most of the free memory
+        needs_grad = any(ctx.needs_input_grad[:7])
+        per_step = 4 * (25 * n + 7 * (cfg.res + 1) ** 3)
+        budget = cfg.record_budget_gib * 2 ** 30
+        if dev.type == "cuda":
+            budget = min(budget, 0.8 * torch.cuda.mem_get_info(dev)[0])
+        if not needs_grad:
+            fwd.run(cfg.nsteps)
+            x, v, _, _ = fwd.state()
+        elif cfg.nsteps > 0 and per_step * cfg.nsteps <= budget:
             # one recorded forward: every substep's state and grids stay on
             # the device for the backward
             after, grids = fwd.replay(cfg.nsteps)

@@ -254,22 +254,45 @@ def baseline_loss(T, w) -> float:
 
 
 class CorrectedRunner:
-    """Corrected big steps with the plain substep as fallback.
+    """Corrected big steps, judged by the plain substep, which is also the
+    fallback.
 
-    Runs in chunks of `check_every` big steps.  After each chunk the state is
-    deemed healthy when no stencil left the grid, every position is finite and
-    the largest particle speed stays below the grid's velocity clamp; an
-    unhealthy chunk is discarded and rerun with factor * check_every plain
-    substeps from its saved start state."""
+    The rollout advances in windows of ``window`` big steps (``window *
+    factor`` plain substeps of simulated time).  A window is *judged* when it
+    is the ``judge_every``-th since the last judged one, or while the runner
+    is on probation after a fallback: the same window is then also run with
+    plain substeps from the window's start state, and the corrected result is
+    kept only if its relative position error against the plain one,
+    |x - x_plain| / |x_plain|, is at most ``tol``; otherwise the plain result
+    replaces it (a fallback) and the next window is judged too.  Every
+    window, judged or not, must also be healthy -- no stencil left the grid,
+    positions finite, speeds below the grid's velocity clamp -- or it is
+    rerun with plain substeps.  ``judge_every = 1`` judges every window (the
+    error bound holds window by window, at the cost of the plain substeps);
+    larger values trade the bound for throughput.  ``stats`` counts the big
+    steps kept, the plain substeps run and the fallbacks."""
 
     def __init__(self, cfg: CorrectorConfig, net: Optional[CorrectorMLP], volume: float, device,
-                 check_every: int = 25):
+                 window: int = 4, judge_every: int = 1, tol: float = 1e-2, check_every: Optional[int] = None):
         self.cfg = cfg
         self.device = torch.device(device)
         self.net = net
         self.volume = volume
-        self.check_every = check_every
-        self.fallbacks = 0
+        self.window = int(check_every or window)
+        self.judge_every = max(1, int(judge_every))
+        self.tol = float(tol)
+        self.stats = dict(windows=0, judged=0, fallbacks=0, unhealthy=0, big_steps_kept=0, plain_substeps=0,
+                          worst_judged_error=0.0)
+
+    @property
+    def fallbacks(self):
+        return self.stats["fallbacks"] + self.stats["unhealthy"]
+
+    def _healthy(self, eng, cur) -> bool:
+        vmax = 0.9 * (1.0 / self.cfg.res) / self.cfg.dt_big
+        if eng.oob_count():
+            return False
+        return bool(torch.isfinite(cur["x"]).all()) and float(cur["v"].norm(dim=1).max()) < vmax
 
     def run(self, s: Dict[str, torch.Tensor], n_big: int, check: bool = True) -> Dict[str, torch.Tensor]:
         cfg = self.cfg
@@ -278,35 +301,50 @@ class CorrectedRunner:
         plain = make_engine(cfg, self.volume, cfg.dt, cfg.gravity, self.device, n) if check else None
         if self.net is not None:
             big.set_corrector(self.net.params, cfg.m_ref, cfg.v_ref, enable=True)
+        st = self.stats
         cur = s
-        done = 0
-        vmax = 0.9 * (1.0 / cfg.res) / cfg.dt_big
+        done, since_judged, probation = 0, 0, False
         try:
-            _upload(big, cur)
+            if not check:
+                _upload(big, cur)
+                big.run(n_big, check_oob=False)
+                st["big_steps_kept"] += n_big
+                return _download(big, cur)
             while done < n_big:
-                k = min(self.check_every, n_big - done)
-                if not check:
-                    big.run(k, check_oob=False)
-                    done += k
-                    continue
-                start = _download(big, cur)
-                ok = True
-                try:
-                    big.run(k, check_oob=True)
-                except RuntimeError:
-                    ok = False
-                if ok:
-                    cur = _download(big, start)
-                    speed = cur["v"].norm(dim=1)
-                    ok = bool(torch.isfinite(cur["x"]).all()) and float(speed.max()) < vmax
+                k = min(self.window, n_big - done)
+                start = cur
+                _upload(big, start)
+                big.run(k, check_oob=False)
+                cand = _download(big, start)
+                st["windows"] += 1
+                since_judged += 1
+                ok = self._healthy(big, cand)
                 if not ok:
-                    self.fallbacks += 1
+                    st["unhealthy"] += 1
+                judge = ok and (probation or since_judged >= self.judge_every)
+                ref = None
+                if judge or not ok:
                     _upload(plain, start)
                     plain.run(k * cfg.factor)
-                    cur = _download(plain, start)
-                    _upload(big, cur)
+                    st["plain_substeps"] += k * cfg.factor
+                    ref = _download(plain, start)
+                if judge:
+                    st["judged"] += 1
+                    since_judged = 0
+                    err = rel_position_error(cand["x"], ref["x"])
+                    st["worst_judged_error"] = max(st["worst_judged_error"], err if math.isfinite(err) else 1.0)
+                    ok = math.isfinite(err) and err <= self.tol
+                    if not ok:
+                        st["fallbacks"] += 1
+                    probation = not ok
+                if ok:
+                    cur = cand
+                    st["big_steps_kept"] += k
+                else:
+                    cur = ref
+                    probation = True
                 done += k
-            return _download(big, cur)
+            return cur
         finally:
             big.close()
             if plain is not None:
@@ -371,6 +409,44 @@ def rollout_report(cfg: CorrectorConfig, net: Optional[CorrectorMLP], device, se
             out[str(k)] = e if math.isfinite(e) else "non-finite"
     finally:
         eng.close()
+    return out
+
+
+def judged_rollout(cfg: CorrectorConfig, net: Optional[CorrectorMLP], device, n_substeps: int = 1000,
+                   settings=((4, 1), (4, 4), (8, 1000000)), tol: float = 1e-2, seed: int = 0) -> Dict:
+    """The corrected form against the plain substep over ``n_substeps`` of
+    simulated time from the scene's t = 0: for each (window, judge_every) the
+    relative position error of the final state against the plain rollout's,
+    the wall time (fallbacks and judging included) and the simulated-time
+    speedup over the plain rollout."""
+    s = scene_tensors(cfg, seed, device)
+    n = s["x"].shape[0]
+    n_big = n_substeps // cfg.factor
+    plain = make_engine(cfg, s["volume"], cfg.dt, cfg.gravity, device, n, sort_every=48)
+    _upload(plain, s)
+    plain.run(n_substeps, check_oob=False)  # captures the graphs
+    _upload(plain, s)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plain.run(n_substeps, check_oob=True)
+    ref = plain.positions().clone()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    plain.close()
+    out = {"substeps": n_substeps, "big_steps": n_big, "tol": tol, "plain_seconds": t_plain, "settings": []}
+    for window, every in settings:
+        runner = CorrectedRunner(cfg, net, s["volume"], device, window=window, judge_every=every, tol=tol)
+        runner.run(s, min(n_big, 2 * window))  # warm-up: engines, graphs
+        runner = CorrectedRunner(cfg, net, s["volume"], device, window=window, judge_every=every, tol=tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fin = runner.run(s, n_big)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        err = rel_position_error(fin["x"], ref)
+        out["settings"].append(dict(window=window, judge_every=every, seconds=t, speedup=t_plain / t,
+                                    rel_position_error=err if math.isfinite(err) else "non-finite",
+                                    within_tol=bool(math.isfinite(err) and err <= tol), **runner.stats))
     return out
 
 

@@ -64,7 +64,9 @@ __device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int aba
     size_t Xi = 0;
     if (!FROM_LDS) Xi = i == 0 ? X[0] : (i == 1 ? X[1] : X[2]);
     const int ai = abase + i * VEL_S0;
-    float Vi[3] = {0.f, 0.f, 0.f}, Tj[3] = {0.f, 0.f, 0.f}, Tk[3] = {0.f, 0.f, 0.f};
+    // accumulators start from their first term (an FMA onto +0 cannot be
+    // folded into a product under IEEE signed zeros)
+    float Vi[3], Tj[3], Tk[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       float s[3], tk[3];
@@ -82,12 +84,18 @@ __device__ __forceinline__ void gather(const float4* __restrict__ vel_s, int aba
           if (k == 2) tk[d] += w2k2 * gg[d];
         }
       }
-      const float wj = w[1][j], jwj = (float)j * w[1][j];
+      const float wj = w[1][j];
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        Vi[d] += wj * s[d];
-        if (j) Tj[d] += jwj * s[d];
-        Tk[d] += wj * tk[d];
+        if (j == 0) {
+          Vi[d] = wj * s[d];
+          Tk[d] = wj * tk[d];
+        } else {
+          Vi[d] += wj * s[d];
+          Tk[d] += wj * tk[d];
+        }
+        if (j == 1) Tj[d] = wj * s[d];
+        if (j == 2) Tj[d] += (2.f * wj) * s[d];
       }
     }
     const float iwi = (float)i * wi;
@@ -637,9 +645,15 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
               const int off = ai + jor[jj] + k;
               // rounded to nearest (fma with 0.5, floor conversion: the
               // same two instructions as a product and a truncation)
-              const int c0 = round_fixed(wt, qij[0] + A2s[0] * (float)k);
-              const int c1 = round_fixed(wt, qij[1] + A2s[1] * (float)k);
-              const int c2 = round_fixed(wt, qij[2] + A2s[2] * (float)k);
+              // q + A2 k for k = 0, 1, 2 (no multiply by a literal 0 or 1,
+              // which IEEE semantics keep from being folded)
+              float qk[3];
+#pragma unroll
+              for (int d = 0; d < 3; ++d)
+                qk[d] = k == 0 ? qij[d] : (k == 1 ? qij[d] + A2s[d] : qij[d] + 2.f * A2s[d]);
+              const int c0 = round_fixed(wt, qk[0]);
+              const int c1 = round_fixed(wt, qk[1]);
+              const int c2 = round_fixed(wt, qk[2]);
               const int c3 = round_fixed(wt, ms);
               atomicAdd(&acc2[0][off], pack_fixed2(c1, c0));
               atomicAdd(&acc2[1][off], pack_fixed2_nn(c2, c3));

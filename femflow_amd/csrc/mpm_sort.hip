@@ -1,6 +1,8 @@
 // Cell binning: key generation, a stable LSD radix sort on 32-bit keys
-// (8-bit digits), a three-phase exclusive scan, and the chunk builder that
-// turns the sorted keys into per-block work items for the transfer kernels.
+// (8-bit digits; three launches per pass: tile histograms, a per-digit scan
+// over tiles, the stable scatter), a three-phase exclusive scan, and the
+// chunk builder that turns the sorted keys into per-block work items for the
+// transfer kernels.
 //
 // Stability: inside a tile each wave owns a contiguous run of keys and walks
 // it in index order; ranks among equal digits come from a 64-lane match mask
@@ -60,18 +62,63 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint32_t* __restr
   hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
 }
 
+// Exclusive prefix sum over the 256 threads of a workgroup (one value each):
+// wave-level inclusive scans by lane shuffles, then the wave totals.
+__device__ __forceinline__ uint32_t excl_scan_256(uint32_t v, uint32_t* wave_tot) {
+  const int t = threadIdx.x;
+  const int lane = t & 63, w = t >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, (unsigned)off);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) wave_tot[w] = incl;
+  __syncthreads();
+  uint32_t base = 0;
+#pragma unroll
+  for (int i = 0; i < RS_WAVES; ++i) base += i < w ? wave_tot[i] : 0u;
+  return base + incl - v;
+}
+
+// One workgroup per digit d: the exclusive prefix of hist[d][*] over tiles
+// (written to offs[d][*]) and the digit's total (dtot[d]).  The scatter adds
+// the exclusive prefix of the totals over digits.
+__global__ __launch_bounds__(RS_THREADS) void rs_digit_scan(const uint32_t* __restrict__ hist, int ntiles,
+                                                           uint32_t* __restrict__ offs,
+                                                           uint32_t* __restrict__ dtot) {
+  __shared__ uint32_t wt[RS_WAVES];
+  const int64_t row = (int64_t)blockIdx.x * ntiles;
+  uint32_t carry = 0;
+  for (int b0 = 0; b0 < ntiles; b0 += RS_THREADS) {
+    const int b = b0 + (int)threadIdx.x;
+    const uint32_t v = b < ntiles ? hist[row + b] : 0u;
+    const uint32_t ex = excl_scan_256(v, wt);
+    if (b < ntiles) offs[row + b] = carry + ex;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int i = 0; i < RS_WAVES; ++i) tot += wt[i];
+    carry += tot;
+    __syncthreads();  // wt is rewritten by the next round
+  }
+  if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
+}
+
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ offs, int ntiles,
-    int iota_values) {
+    uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ offs,
+    const uint32_t* __restrict__ dtot, int ntiles, int iota_values) {
   __shared__ uint32_t cnt[RS_WAVES][256];
   __shared__ uint32_t goff[256];
+  __shared__ uint32_t wt[RS_WAVES];
   const int t = threadIdx.x;
   const int w = t >> 6;
   const unsigned lane = __lane_id();
 #pragma unroll
   for (int i = 0; i < RS_WAVES; ++i) cnt[i][t] = 0;
-  goff[t] = offs[(int64_t)t * ntiles + blockIdx.x];
+  // global start of digit t in this tile: keys of smaller digits (all tiles)
+  // plus keys of digit t in earlier tiles
+  goff[t] = excl_scan_256(dtot[t], wt) + offs[(int64_t)t * ntiles + blockIdx.x];
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * 64 * RS_ITEMS;
   uint32_t k[RS_ITEMS];
@@ -222,8 +269,8 @@ size_t radix_workspace_bytes(int64_t n) {
   const int64_t ntiles = cdiv(n > 0 ? n : 1, RS_TILE);
   const int64_t hist = 256 * ntiles;
   const int64_t parts = cdiv(hist > n ? hist : n, SCAN_TILE) + 1;
-  // keys/vals ping-pong (4 arrays of n) + hist + offs + partials + total
-  return sizeof(uint32_t) * (4 * (size_t)n + 2 * (size_t)(hist > n ? hist : n) + parts + 64);
+  // keys/vals ping-pong (4 arrays of n) + hist + offs + partials + digit totals
+  return sizeof(uint32_t) * (4 * (size_t)n + 2 * (size_t)(hist > n ? hist : n) + parts + 256 + 64);
 }
 
 void exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* part,
@@ -254,16 +301,16 @@ void radix_sort(uint32_t* keys, uint32_t* vals, int64_t n, uint32_t max_key, voi
   uint32_t* v2 = w + n;
   uint32_t* hist = w + 2 * n;
   uint32_t* offs = hist + big;
-  uint32_t* part = offs + big;
+  uint32_t* dtot = offs + big;
   uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
   const int passes = radix_passes(max_key);
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     hipLaunchKernelGGL(rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, n, shift,
                        hist, (int)ntiles);
-    exclusive_scan(hist, offs, hist_n, part, nullptr, st);
+    hipLaunchKernelGGL(rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, st, hist, (int)ntiles, offs, dtot);
     hipLaunchKernelGGL(rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, va, kb, vb,
-                       n, shift, offs, (int)ntiles, (iota_values && p == 0) ? 1 : 0);
+                       n, shift, offs, dtot, (int)ntiles, (iota_values && p == 0) ? 1 : 0);
     uint32_t* t;
     t = ka; ka = kb; kb = t;
     t = va; va = vb; vb = t;

@@ -204,32 +204,6 @@ def test_rollout_gradient_matches_fp64_reference(gpu, budget):
         assert e < 5e-3, name
 
 
-def _fp64_gpu_loss(gpu, x0, v0, mu0, lam0, mass, vol, res, dt, steps, sub, target):
-    """FP64 dense-grid forward on the GPU (the library's FP64 phase kernels)."""
-    from femflow_amd import _lib
-    from femflow_amd.solvers.mpm import _phase
-
-    n = x0.shape[0]
-    f64 = lambda a: torch.as_tensor(a, dtype=torch.float64, device=gpu).clone().contiguous()  # noqa: E731
-    x, v = f64(x0), f64(v0)
-    F = torch.eye(3, dtype=torch.float64, device=gpu).repeat(n, 1, 1)
-    C = torch.zeros((n, 3, 3), dtype=torch.float64, device=gpu)
-    Jp = torch.ones(n, dtype=torch.float64, device=gpu)
-    sp = _phase.StagedParticles(x, v, F, C, Jp, f64(mass), f64(mu0), f64(lam0))
-    prm = _phase.make_params(3, res, float(res), 1.0 / res, dt, vol, -9.8)
-    r1 = res + 1
-    gv = torch.zeros((r1, r1, r1, 3), dtype=torch.float64, device=gpu)
-    gm = torch.zeros((r1, r1, r1, 1), dtype=torch.float64, device=gpu)
-    for _ in range(steps):
-        gv.zero_()
-        gm.zero_()
-        assert _phase.run_p2g(sp, prm, _lib.MPM_ELASTIC, 1.0, gv, gm) == 0
-        _phase.run_grid_op(prm, gv, gm)
-        assert _phase.run_g2p(sp, prm, _lib.MPM_ELASTIC, gv) == 0
-    c = x[torch.as_tensor(sub, device=gpu)].mean(0)
-    return float(((c - torch.as_tensor(target, dtype=torch.float64, device=gpu)) ** 2).sum())
-
-
 def full_size_adjoint_case(seed=0):
     """The full-size gradient workload: 2^17 particles uniform in the cube
     [0.4, 0.6]^3 on a 64^3 grid, E ~ U[1e3, 1e4] per particle (nu = 0.2),
@@ -251,8 +225,36 @@ def full_size_adjoint_case(seed=0):
                 mass=np.full(n, vol), rng=rng)
 
 
+def _c_oracle_losses(x0, v0, E, mass, vol, res, dt, steps, sub, tgt_sub, tgt_all):
+    """Both centroid losses after an FP64 rollout of the C restatement of the
+    reference step (every host thread): the upper-half centroid's and the
+    whole-body centroid's (the configuration's literal loss)."""
+    import os
+
+    from oracle import c_oracle as CO
+
+    n = len(x0)
+    mu, lam = O.Ev_to_mu(E, 0.2), O.Ev_to_lambda(E, 0.2)
+    st = dict(x=x0.copy(), v=v0.copy(), F=np.broadcast_to(np.eye(3), (n, 3, 3)).copy(), C=np.zeros((n, 3, 3)),
+              Jp=np.ones((n, 1)), mass=mass, mu0=mu, lam0=lam, kind=np.zeros(n, np.int32), h=np.ones(n),
+              theta_c=np.full(n, 2.5e-2), theta_s=np.full(n, 7.5e-3))
+    pr = dict(res=res, inv_dx=float(res), dx=1.0 / res, dt=dt, volume=vol, gravity=-9.8)
+    try:
+        nt = max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        nt = 1
+    xT = CO.run(st, pr, steps, nthreads=nt).arrays["x"]
+    return float(((xT[sub].mean(0) - tgt_sub) ** 2).sum()), float(((xT.mean(0) - tgt_all) ** 2).sum())
+
+
 @pytest.mark.gpu
 def test_rollout_full_size_matches_central_differences(gpu):
+    """The full-size case against directional central differences (h = 1e-4)
+    of the FP64 C restatement of the reference step, for the upper-half
+    centroid loss (v0 and per-particle E) and for the configuration's literal
+    whole-body centroid loss (v0; its E gradient vanishes, see
+    full_size_adjoint_case).  Bars: 1e-4 relative for v0, 2e-3 for E (about
+    ten times the agreement measured)."""
     import time
 
     from femflow_amd.adjoint import RolloutConfig, lame_from_E, rollout
@@ -263,32 +265,37 @@ def test_rollout_full_size_matches_central_differences(gpu):
     f32 = lambda a: torch.tensor(np.asarray(a), dtype=torch.float32, device=gpu)  # noqa: E731
     v0t, Et = f32(c["v0"]).requires_grad_(True), f32(c["E"]).requires_grad_(True)
     sub = torch.as_tensor(c["sub"], device=gpu)
+    tgt_all = c["x0"].mean(0) + 0.05 * np.random.default_rng(1).standard_normal(3)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     mu0, lam0 = lame_from_E(Et, 0.2)
     xT, _ = rollout(f32(c["x0"]), v0t, mu0, lam0, f32(c["mass"]), cfg)
     L = ((xT[sub].mean(0) - f32(c["target"])) ** 2).sum()
-    gv, gE = torch.autograd.grad(L, (v0t, Et))
+    L_all = ((xT.mean(0) - f32(tgt_all)) ** 2).sum()
+    gv, gE = torch.autograd.grad(L, (v0t, Et), retain_graph=True)
+    (gv_all,) = torch.autograd.grad(L_all, (v0t,))
     torch.cuda.synchronize()
     print(f"full-size rollout forward+backward: {time.perf_counter() - t0:.2f} s, loss {float(L):.6e}", flush=True)
-    # directional central differences of the FP64 forward (same FP32 inputs), h = 1e-4
+    # the oracle sees the FP32 inputs the GPU saw
     x0_, v0_ = f32(c["x0"]).double().cpu().numpy(), v0t.detach().double().cpu().numpy()
     E_, m_ = Et.detach().double().cpu().numpy(), f32(c["mass"]).double().cpu().numpy()
     tgt = f32(c["target"]).double().cpu().numpy()
+    tga = f32(tgt_all).double().cpu().numpy()
     h = 1e-4
 
-    def loss64(v0, E):
-        mu, lam = lame_from_E(torch.as_tensor(E), 0.2)
-        return _fp64_gpu_loss(gpu, x0_, v0, mu, lam, m_, vol, res, dt, steps, c["sub"], tgt)
+    def losses(v0, E):
+        return _c_oracle_losses(x0_, v0, E, m_, vol, res, dt, steps, c["sub"], tgt, tga)
 
     d = rng.standard_normal((n, 3))
     d /= np.linalg.norm(d)
-    fd = (loss64(v0_ + h * d, E_) - loss64(v0_ - h * d, E_)) / (2 * h)
-    ad = float((gv.double().cpu().numpy() * d).sum())
-    print(f"dL/dv0 . d: adjoint {ad:.6e} fd {fd:.6e}", flush=True)
-    assert abs(ad - fd) <= 2e-2 * abs(fd)
+    lp, lm = losses(v0_ + h * d, E_), losses(v0_ - h * d, E_)
+    for name, g, k in (("upper half", gv, 0), ("whole body", gv_all, 1)):
+        fd = (lp[k] - lm[k]) / (2 * h)
+        ad = float((g.double().cpu().numpy() * d).sum())
+        print(f"dL/dv0 . d [{name}]: adjoint {ad:.6e} fd {fd:.6e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+        assert abs(ad - fd) <= 1e-4 * abs(fd), name
     for name, dE in (("uniform", E_.copy()), ("random", E_ * rng.standard_normal(n))):
-        fd = (loss64(v0_, E_ + h * dE) - loss64(v0_, E_ - h * dE)) / (2 * h)
+        fd = (losses(v0_, E_ + h * dE)[0] - losses(v0_, E_ - h * dE)[0]) / (2 * h)
         ad = float((gE.double().cpu().numpy() * dE).sum())
-        print(f"dL/dE . d[{name}]: adjoint {ad:.6e} fd {fd:.6e}", flush=True)
-        assert abs(ad - fd) <= 5e-2 * abs(fd), name
+        print(f"dL/dE . d [{name}]: adjoint {ad:.6e} fd {fd:.6e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+        assert abs(ad - fd) <= 2e-3 * abs(fd), name

@@ -69,3 +69,54 @@ def test_autotune_restores_state_and_caches(gpu, tmp_path):
     again = run.autotune(cache_path=str(cache))
     assert again["cached"] and again["config"] == res["config"]
     run.close()
+
+
+@pytest.mark.gpu
+def test_full_size_batch_matches_oracle_on_selected_scenes(gpu):
+    """The batched configuration at full size (64 scenes, seed 0, ~5M
+    particles, 64^3 to 128^3 grids) in one engine, 4 substeps with the
+    benchmark's launch shape; the smallest, median and largest scene and the
+    largest scene of each material against the C restatement of the reference
+    step (FP64, every host thread), from the FP32 values the GPU saw."""
+    import os
+
+    import torch
+
+    from femflow_amd.batched import BatchedRun, build_batched
+    from oracle import c_oracle as CO
+    from oracle import mpm_oracle as O
+    from tests._parity import field_errors
+
+    work = build_batched(n_scenes=64, seed=0, keep_states=True)
+    counts = np.array(work["counts"])
+    assert counts.sum() > 4_900_000
+    run = BatchedRun(work, device=gpu)
+    run.engine.tune(chunk=512, grid_mult=8, sort_every=48, spread=9)
+    steps = 4
+    run.run(steps)
+    out = run.download()
+    torch.cuda.synchronize()
+    order = np.argsort(counts, kind="stable")
+    pick = {int(order[0]): "smallest", int(order[len(order) // 2]): "median", int(order[-1]): "largest"}
+    mats = np.array([s["material"] for s in work["specs"]])
+    for m in (0, 1, 2):
+        idx = np.nonzero(mats == m)[0]
+        pick.setdefault(int(idx[np.argmax(counts[idx])]), f"largest of material {m}")
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    try:
+        nt = max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        nt = 1
+    for i, why in sorted(pick.items()):
+        st, pr = work["states"][i], work["params"][i]
+        a, b = offs[i], offs[i + 1]
+        ref = O.copy_state(st)
+        for k in ("x", "v", "F", "C", "Jp", "mass", "mu0", "lam0"):
+            ref[k] = run.t[k][a:b].double().cpu().numpy().reshape(st[k].shape)
+        got = CO.run(ref, pr, steps, nthreads=nt).to_state()
+        errs = field_errors({k: v[a:b] for k, v in out.items()}, got, pr["inv_dx"])
+        print(f"scene {i} ({why}): n={b - a} res={pr['res']} material={mats[i]}", errs, flush=True)
+        for k in ("x", "v", "F"):
+            assert errs[k] < 1e-4, (i, why, errs)
+    assert run.engine.oob_count() == 0
+    run.close()
