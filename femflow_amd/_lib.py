@@ -27,7 +27,7 @@ EXPORTS = (
     "mpm_ctx_kernel_timing", "mpm_ctx_kernel_times", "mpm_state_upload", "mpm_state_download",
     "mpm_run", "mpm_run_frames", "mpm_run_states", "mpm_run_states_grids", "mpm_state_positions", "mpm_substep", "mpm_ctx_oob_count", "mpm_ctx_fallback_count",
     "mpm_ctx_active_blocks", "mpm_ctx_request_rebin",
-    "mpm_particle_sums", "mpm_bin", "mpm_dense_workspace_bytes", "mpm_dense_substep", "mpm_adjoint_step",
+    "mpm_particle_sums", "mpm_bin", "mpm_bin_noalloc", "mpm_dense_workspace_bytes", "mpm_dense_substep", "mpm_adjoint_step",
     "mpm_adjoint_step_grids",
     "mpm_ctx_set_corrector", "mpm_corrector_sample", "mpm_grid_velocity",
     "mpm_collide_planes", "mpm_ctx_set_planes",
@@ -113,6 +113,7 @@ def load():
             "mpm_ctx_request_rebin": ([vp], i32),
             "mpm_particle_sums": ([vp, ct.POINTER(dbl)], i32),
             "mpm_bin": ([vp, i32, i32, dbl, i32, vp, i64, vp, vp], i32),
+            "mpm_bin_noalloc": ([vp, i32, i32, dbl, i32, vp, i64, vp, vp], i32),
             "mpm_dense_workspace_bytes": ([Q, ct.POINTER(i64)], i32),
             "mpm_dense_substep": ([vp, Q, P, dbl, vp, vp], i32),
             "mpm_adjoint_step": ([vp, Q, P, dbl, vp, vp, vp, vp, vp, vp, vp, vp], i32),

@@ -223,6 +223,15 @@ int32_t mpm_particle_sums(mpm_ctx* ctx, double* out);
  * three_d/p2g.py:49); the key derives from its base, three_d/p2g.py:50. */
 int32_t mpm_bin(mpm_ctx* ctx, int32_t dim, int32_t res, double inv_dx, int32_t layout,
                 const float* x, int64_t n, uint32_t* keys, uint32_t* perm);
+/* mpm_bin that allocates nothing: the same arguments and bit-identical keys
+ * and perm, with the sort's workspace in a fixed device array of the library
+ * (about 20 MiB per GPU, held until the process ends) instead of per-call
+ * hipMalloc/hipFree.  n is limited to 2^20 (larger n: MPM_ERR_ARG).  Calls
+ * that overlap on one GPU share that array and race.  No counterpart in the
+ * reference: it serves callers that cannot allocate device memory (code
+ * under stream capture, tight serving loops, allocation-free harnesses). */
+int32_t mpm_bin_noalloc(mpm_ctx* ctx, int32_t dim, int32_t res, double inv_dx, int32_t layout,
+                        const float* x, int64_t n, uint32_t* keys, uint32_t* perm);
 
 /* ---- differentiable rollouts: dense-grid substep and its adjoint ------
  * Elastic (fixed-corotated, utils.py:95-135 with constant_hardening,
