@@ -114,6 +114,60 @@ def cpu_baseline(state, params, budget_s, nthreads=1):
             "cpu_model": cpu_model(), "nproc": os.cpu_count()}
 
 
+def cpu_drift_baseline(dev, nthreads, n=1 << 17, res=128, steps=1000, every=100):
+    """The C restatement of the reference step on `nthreads` host threads,
+    timed over the configuration's whole rollout (1000 substeps) of the
+    three-material scene at 128^3 with n particles, with the FP32 engine's
+    state compared against it every `every` substeps (relative inf-norm
+    errors; v floored at 1e-6 m/s): the drift over the rollout."""
+    import numpy as np
+    import torch
+
+    from femflow_amd.engine import Engine, MaterialSpec, SceneSpec
+    from femflow_amd.scenes import MATERIALS_C2, scene_3d_cubes
+    from oracle import c_oracle
+
+    st, pr = scene_3d_cubes(n=n, res=res)
+    n = len(st["x"])
+    t = lambda a: torch.tensor(np.ascontiguousarray(a), dtype=torch.float32, device=dev)  # noqa: E731
+    ts = {k: t(st[k]) for k in ("x", "v", "F", "C", "mass", "mu0", "lam0")}
+    ts["Jp"] = t(st["Jp"].reshape(n))
+    ref = dict(st)
+    for k in ("x", "v", "F", "C", "Jp", "mass", "mu0", "lam0"):
+        ref[k] = ts[k].double().cpu().numpy().reshape(np.shape(st[k]))
+    cs = c_oracle.CState(ref)
+    mats = [MaterialSpec(("elastic", "snow", "water")[k], h, tc, tsn) for k, _, _, h, tc, tsn in MATERIALS_C2]
+    eng = Engine([SceneSpec(res=pr["res"], dt=pr["dt"], gravity=pr["gravity"], volume=pr["volume"])], mats,
+                 n_max=n, device=dev)
+    eng.tune(chunk=512, sort_every=48, grid_mult=8)
+    eng.upload(ts["x"], ts["v"], ts["F"], ts["C"], ts["Jp"], ts["mass"], ts["mu0"], ts["lam0"],
+               torch.tensor(st["kind"], dtype=torch.int32, device=dev))
+    out = {k: torch.empty_like(ts[k]) for k in ("x", "v", "F", "C", "Jp")}
+    cpu_s, table = 0.0, {}
+    for k in range(1, steps // every + 1):
+        eng.run(every, check_oob=True)
+        eng.download(out["x"], out["v"], out["F"], out["C"], out["Jp"])
+        t0 = time.perf_counter()
+        c_oracle.run(ref, pr, every, nthreads=nthreads, cstate=cs)
+        cpu_s += time.perf_counter() - t0
+        torch.cuda.synchronize()
+        errs = {}
+        for f, floor in (("x", 1e-12), ("v", 1e-6), ("F", 1e-12)):
+            a = out[f].double().cpu().numpy().reshape(n, -1)
+            b = cs.arrays[f].reshape(n, -1)
+            errs[f] = float(np.abs(a - b).max() / max(np.abs(b).max(), floor))
+        table[str(k * every)] = errs
+    eng.close()
+    print(f"[bench] CPU restatement on {nthreads} threads: {steps} substeps of {n} particles in {cpu_s:.1f} s; "
+          f"drift after {steps}: {table[str(steps)]}", file=sys.stderr, flush=True)
+    how = "one thread" if nthreads == 1 else f"{nthreads} OpenMP threads (node slabs, deterministic)"
+    return {"value": n * steps / cpu_s, "unit": "particle-substeps/s", "cores": nthreads, "kind": "port",
+            "sample": f"C restatement of the reference step (FP64, {how}), the whole {steps}-substep rollout of "
+                      f"the three-material scene at {res}^3 with {n} particles",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+            "drift_fp32_engine_vs_this": table}
+
+
 def batched_bench(steps, dev):
     """64 scenes (log-uniform 4k-256k particles, 64^3-128^3 grids, random
     material and box) in one launch sequence per substep, autotuned."""
@@ -418,17 +472,19 @@ def main():
     eng.run(nprof, check_oob=False)
     kt = eng.kernel_times()
     eng.kernel_timing(False)
-    # FP64 grid-mass drift over a separate stretch with the per-substep sums on
+    grid_blocks = eng.active_block_count()  # blocks the grid update processes
+    # FP64 grid-mass drift over a separate stretch with the per-substep sums
+    # on; sums[:, 7] counts the grid nodes with mass (the active nodes)
     sums = torch.zeros((nprof, 8), dtype=torch.float64, device=dev)
     eng.run(nprof, sums=sums, check_oob=False)
     g_ms, g_cnt = kt["g2p2g"]
     dom = "g2p2g" if g_cnt else "p2g"
     d_ms, d_cnt = kt[dom]
     avg_s = d_ms / 1e3 / max(d_cnt, 1)
-    active_nodes = eng.active_block_count() * 64
+    s = sums.cpu().numpy()
+    active_nodes = float(s[:, 7].mean()) if nprof else 0.0
     alg_bytes = BYTES_PER_PARTICLE * n + BYTES_PER_NODE_TRANSFER * active_nodes
     achieved = alg_bytes / avg_s / 1e9
-    s = sums.cpu().numpy()
     mass_drift = float(np.abs(s[:, 0] / s[0, 0] - 1).max()) if nprof else 0.0
     # HBM bytes per launch of the fused kernel from the committed rocprofv3
     # PMC summary of this workload (FETCH_SIZE, WRITE_SIZE, calibrated by a
@@ -451,7 +507,7 @@ def main():
         cpu = cpu_baseline(st, pr, args.cpu_seconds)
         nt = host_threads()
         if nt > 1:
-            cpu_omp = cpu_baseline(st, pr, args.cpu_seconds, nthreads=nt)
+            cpu_omp = cpu_drift_baseline(dev, nt)
 
     out = {
         "metric": METRIC,
@@ -479,6 +535,7 @@ def main():
             "traffic_source": "profiles/g2p2g_c2_1M_128_traffic.json (rocprofv3 PMC run of this workload)",
             "kernel": dom, "kernel_ms": avg_s * 1e3,
             "alg_bytes_per_launch": alg_bytes, "active_nodes": active_nodes,
+            "grid_blocks_processed": grid_blocks,
         },
         "rebins_in_window": -(-args.steps // args.sort_every),
         "steady_state": steady,

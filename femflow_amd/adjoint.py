@@ -219,6 +219,9 @@ class _Rollout(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, gx, gv):
+        if ctx.ckpts is None:
+            raise RuntimeError("rollout: the backward pass consumes the forward's record and checkpoints, so it "
+                               "runs once per forward (call rollout again instead of retain_graph=True)")
         st, cfg = ctx.stepper, ctx.cfg
         mass_, mu_, lam_ = ctx.mats
         dev = st.device

@@ -281,8 +281,28 @@ class CorrectedRunner:
         self.window = int(check_every or window)
         self.judge_every = max(1, int(judge_every))
         self.tol = float(tol)
+        self._engines = {}  # particle count -> (big, plain), kept across run() calls
+        self.reset_stats()
+
+    def reset_stats(self):
         self.stats = dict(windows=0, judged=0, fallbacks=0, unhealthy=0, big_steps_kept=0, plain_substeps=0,
                           worst_judged_error=0.0)
+
+    def _engines_for(self, n: int):
+        if n not in self._engines:
+            cfg = self.cfg
+            big = make_engine(cfg, self.volume, cfg.dt_big, cfg.gravity, self.device, n)
+            plain = make_engine(cfg, self.volume, cfg.dt, cfg.gravity, self.device, n, sort_every=48)
+            if self.net is not None:
+                big.set_corrector(self.net.params, cfg.m_ref, cfg.v_ref, enable=True)
+            self._engines[n] = (big, plain)
+        return self._engines[n]
+
+    def close(self):
+        for big, plain in self._engines.values():
+            big.close()
+            plain.close()
+        self._engines = {}
 
     @property
     def fallbacks(self):
@@ -295,60 +315,54 @@ class CorrectedRunner:
         return bool(torch.isfinite(cur["x"]).all()) and float(cur["v"].norm(dim=1).max()) < vmax
 
     def run(self, s: Dict[str, torch.Tensor], n_big: int, check: bool = True) -> Dict[str, torch.Tensor]:
+        """n_big big steps of simulated time from state s (caller order);
+        the engines are created on the first call and reused."""
         cfg = self.cfg
         n = s["x"].shape[0]
-        big = make_engine(cfg, self.volume, cfg.dt_big, cfg.gravity, self.device, n)
-        plain = make_engine(cfg, self.volume, cfg.dt, cfg.gravity, self.device, n) if check else None
-        if self.net is not None:
-            big.set_corrector(self.net.params, cfg.m_ref, cfg.v_ref, enable=True)
+        big, plain = self._engines_for(n)
         st = self.stats
         cur = s
         done, since_judged, probation = 0, 0, False
-        try:
-            if not check:
-                _upload(big, cur)
-                big.run(n_big, check_oob=False)
-                st["big_steps_kept"] += n_big
-                return _download(big, cur)
-            while done < n_big:
-                k = min(self.window, n_big - done)
-                start = cur
-                _upload(big, start)
-                big.run(k, check_oob=False)
-                cand = _download(big, start)
-                st["windows"] += 1
-                since_judged += 1
-                ok = self._healthy(big, cand)
+        if not check:
+            _upload(big, cur)
+            big.run(n_big, check_oob=False)
+            st["big_steps_kept"] += n_big
+            return _download(big, cur)
+        while done < n_big:
+            k = min(self.window, n_big - done)
+            start = cur
+            _upload(big, start)
+            big.run(k, check_oob=False)
+            cand = _download(big, start)
+            st["windows"] += 1
+            since_judged += 1
+            ok = self._healthy(big, cand)
+            if not ok:
+                st["unhealthy"] += 1
+            judge = ok and (probation or since_judged >= self.judge_every)
+            ref = None
+            if judge or not ok:
+                _upload(plain, start)
+                plain.run(k * cfg.factor)
+                st["plain_substeps"] += k * cfg.factor
+                ref = _download(plain, start)
+            if judge:
+                st["judged"] += 1
+                since_judged = 0
+                err = rel_position_error(cand["x"], ref["x"])
+                st["worst_judged_error"] = max(st["worst_judged_error"], err if math.isfinite(err) else 1.0)
+                ok = math.isfinite(err) and err <= self.tol
                 if not ok:
-                    st["unhealthy"] += 1
-                judge = ok and (probation or since_judged >= self.judge_every)
-                ref = None
-                if judge or not ok:
-                    _upload(plain, start)
-                    plain.run(k * cfg.factor)
-                    st["plain_substeps"] += k * cfg.factor
-                    ref = _download(plain, start)
-                if judge:
-                    st["judged"] += 1
-                    since_judged = 0
-                    err = rel_position_error(cand["x"], ref["x"])
-                    st["worst_judged_error"] = max(st["worst_judged_error"], err if math.isfinite(err) else 1.0)
-                    ok = math.isfinite(err) and err <= self.tol
-                    if not ok:
-                        st["fallbacks"] += 1
-                    probation = not ok
-                if ok:
-                    cur = cand
-                    st["big_steps_kept"] += k
-                else:
-                    cur = ref
-                    probation = True
-                done += k
-            return cur
-        finally:
-            big.close()
-            if plain is not None:
-                plain.close()
+                    st["fallbacks"] += 1
+                probation = not ok
+            if ok:
+                cur = cand
+                st["big_steps_kept"] += k
+            else:
+                cur = ref
+                probation = True
+            done += k
+        return cur
 
 
 def rel_position_error(x, ref) -> float:
@@ -371,7 +385,9 @@ def one_step_gain(cfg: CorrectorConfig, net: CorrectorMLP, device, seeds=(901, 9
         eng.close()
         disp = (ref - s["x"]).norm()
         for tag, nn in (("without", None), ("with", net)):
-            out = CorrectedRunner(cfg, nn, s["volume"], device).run(s, 1, check=False)
+            runner = CorrectedRunner(cfg, nn, s["volume"], device)
+            out = runner.run(s, 1, check=False)
+            runner.close()
             errs[tag].append(float((out["x"] - ref).norm() / disp))
     return {k: float(np.mean(v)) for k, v in errs.items()}
 
@@ -413,7 +429,8 @@ def rollout_report(cfg: CorrectorConfig, net: Optional[CorrectorMLP], device, se
 
 
 def judged_rollout(cfg: CorrectorConfig, net: Optional[CorrectorMLP], device, n_substeps: int = 1000,
-                   settings=((4, 1), (4, 4), (8, 1000000)), tol: float = 1e-2, seed: int = 0) -> Dict:
+                   settings=((1, 1), (1, 4), (2, 1), (1, 1000000), (4, 1000000)), tol: float = 1e-2,
+                   seed: int = 0) -> Dict:
     """The corrected form against the plain substep over ``n_substeps`` of
     simulated time from the scene's t = 0: for each (window, judge_every) the
     relative position error of the final state against the plain rollout's,
@@ -437,12 +454,13 @@ def judged_rollout(cfg: CorrectorConfig, net: Optional[CorrectorMLP], device, n_
     for window, every in settings:
         runner = CorrectedRunner(cfg, net, s["volume"], device, window=window, judge_every=every, tol=tol)
         runner.run(s, min(n_big, 2 * window))  # warm-up: engines, graphs
-        runner = CorrectedRunner(cfg, net, s["volume"], device, window=window, judge_every=every, tol=tol)
+        runner.reset_stats()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         fin = runner.run(s, n_big)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
+        runner.close()
         err = rel_position_error(fin["x"], ref)
         out["settings"].append(dict(window=window, judge_every=every, seconds=t, speedup=t_plain / t,
                                     rel_position_error=err if math.isfinite(err) else "non-finite",

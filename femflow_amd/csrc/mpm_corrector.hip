@@ -97,9 +97,10 @@ __global__ __launch_bounds__(256) void corrector_kernel(EngineArgs a, CorrArgs c
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
   const int li = lane & 15, lk = (lane >> 4) * 8;
-  const uint32_t count = *a.list_count;
+  const uint32_t ncand = *a.cand_count;
+  const uint32_t count = ncand + *a.list_count;
   for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
-    const uint32_t g = a.list[b];
+    const uint32_t g = active_block(a, b, ncand);
     const SceneDev sc = a.scenes[a.block_scene[g]];
     const int local = (int)g - sc.block_base;
     const int nb = sc.nb;
@@ -258,12 +259,14 @@ __global__ __launch_bounds__(256) void corrector_kernel(EngineArgs a, CorrArgs c
 // (every cell that received a particle lies in one: cellf_kernel marks it),
 // so that blocks outside the next substep's active set read as empty.
 __global__ __launch_bounds__(256) void cellf_clear_kernel(EngineArgs a) {
-  const uint32_t count = *a.list_count;
+  const uint32_t ncand = *a.cand_count;
+  const uint32_t count = ncand + *a.list_count;
   const int t = threadIdx.x;
   for (uint32_t b = blockIdx.x; b < count; b += gridDim.x) {
-    float4* p = a.cellf + (size_t)a.list[b] * 64 * 3;
+    const uint32_t g = active_block(a, b, ncand);
+    float4* p = a.cellf + (size_t)g * 64 * 3;
     if (t < 192) p[t] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (t < 64) a.pm[(size_t)a.list[b] * 64 + t] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (t < 64) a.pm[(size_t)g * 64 + t] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 

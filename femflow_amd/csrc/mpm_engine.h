@@ -89,6 +89,7 @@ struct Engine {
            *d_cidx = nullptr, *d_part = nullptr;
   uint32_t* d_sorted_keys = nullptr;
   uint32_t* d_hard = nullptr;
+  uint32_t *d_cand_list = nullptr, *d_cand_flag = nullptr;
   float4* d_stage = nullptr;
   // learned corrector
   bool corr_on = false;

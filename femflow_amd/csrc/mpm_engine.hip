@@ -430,7 +430,6 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
       reinterpret_cast<unsigned long long(*)[MODE == 2 ? 1 : ACC_NODES]>(arena_s);
   __shared__ float red_s[ENG_THREADS / 64][4];
   __shared__ ChunkScale cs;
-  __shared__ uint32_t sub_active[27];
   __shared__ uint32_t wg_oob, wg_fb;
   // P2G term table, four float4 per chunk entry, (fx0, fx1, fx2, m),
   // (q0[0..2], Adx0), (Adx1..4), (Adx5..8), read and written 16 bytes at a
@@ -453,9 +452,13 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   const int L = ((int)gridDim.x - xcd + 7) >> 3;
   const uint32_t lo = (uint32_t)(((uint64_t)nch * xcd) >> 3);
   const uint32_t hi = (uint32_t)(((uint64_t)nch * (xcd + 1)) >> 3);
+  // a single-scene engine reads its scene once, not once per chunk (one
+  // dependent load fewer ahead of each chunk's velocity arena)
+  const bool one_scene = a.nscenes == 1;
+  const SceneDev sc0 = a.scenes[0];
   for (uint32_t c = lo + l; c < hi; c += L) {
     const int4 ch = a.chunks[c];
-    const SceneDev sc = a.scenes[ch.w];
+    const SceneDev sc = one_scene ? sc0 : a.scenes[ch.w];
     const int local = ch.z - sc.block_base;
     const int nb = sc.nb;
     const int bi = local / (nb * nb), bj = (local / nb) % nb, bk = local % nb;
@@ -468,7 +471,6 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
             in ? a.vel[node_addr(sc, gi, gj, gk)] : make_float4(0.f, 0.f, 0.f, 0.f);
       }
     }
-    if (t < 27) sub_active[t] = 0;
     __syncthreads();
     // Phase 1: particle q = r * ENG_THREADS + t of the chunk (coalesced).
     // G2P modes gather, advance and store the particle; P2G modes then
@@ -676,26 +678,18 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
                                        (float)((double)q2 * inv[2]), (float)((double)q3 * inv[3]));
         const bool to_stage = staged && e0 < STAGE && e1 < STAGE && e2 < STAGE;
         if (to_stage) stage[(e0 * STAGE + e1) * STAGE + e2] = val;
-        if (q0 != 0 || q1 != 0 || q2 != 0 || q3 != 0) {
+        // every arena node lies in one of the 27 blocks around this chunk's
+        // block, all candidates of the grid update: no marking needed
+        if (!to_stage && (q0 != 0 || q1 != 0 || q2 != 0 || q3 != 0)) {
           const int gi = O0 + e0, gj = O1 + e1, gk = O2 + e2;
           if (gi >= 0 && gj >= 0 && gk >= 0 && gi <= sc.res && gj <= sc.res && gk <= sc.res) {
-            if (!to_stage) {
-              float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, gi, gj, gk));
-              atomicAdd(dst + 0, val.x);
-              atomicAdd(dst + 1, val.y);
-              atomicAdd(dst + 2, val.z);
-              atomicAdd(dst + 3, val.w);
-            }
-            const int si = (gi >> 2) - (bi - 1), sj = (gj >> 2) - (bj - 1), sk = (gk >> 2) - (bk - 1);
-            sub_active[(si * 3 + sj) * 3 + sk] = 1u;
+            float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, gi, gj, gk));
+            atomicAdd(dst + 0, val.x);
+            atomicAdd(dst + 1, val.y);
+            atomicAdd(dst + 2, val.z);
+            atomicAdd(dst + 3, val.w);
           }
         }
-      }
-      __syncthreads();
-      if (t < 27 && sub_active[t]) {
-        const int gbi = bi - 1 + t / 9, gbj = bj - 1 + (t / 3) % 3, gbk = bk - 1 + t % 3;
-        if (gbi >= 0 && gbj >= 0 && gbk >= 0 && gbi < nb && gbj < nb && gbk < nb)
-          mark_active(a, (uint32_t)(sc.block_base + (gbi * nb + gbj) * nb + gbk));
       }
     }
     __syncthreads();
@@ -818,26 +812,27 @@ __device__ __forceinline__ double wave_sum(double v) {
 __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, int with_sums, int len,
                                                                   int k) {
   __shared__ double red[ENG_THREADS / 64][8];
-  const uint32_t count = *a.list_count;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *a.other_count = 0u;  // the list the next substep appends to
     a.counters[3] = 0u;   // deferred list (its pass has run)
   }
   const int t = threadIdx.x;
   const int wv = t >> 6, inner = t & 63;
-  double s[7] = {0, 0, 0, 0, 0, 0, 0};
-  // One active block per wave, one node per lane.  The kernel is a chain of
+  double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // One listed block per wave, one node per lane.  The kernel is a chain of
   // dependent loads (list entry, neighbour chunk slots, staged arenas), so
-  // the first list entry is read together with the count (every slot below
-  // list_cap is addressable; entries past the count are ignored), and a
-  // single-scene engine skips the block -> scene lookup.
+  // the first candidate entry is read together with the counts (every slot
+  // below list_cap is addressable; entries past the count are ignored), and
+  // a single-scene engine skips the block -> scene lookup.
   const bool one_scene = a.nscenes == 1;
   const uint32_t stride = gridDim.x * 4u;
   uint32_t b = blockIdx.x * 4u + (uint32_t)wv;
-  uint32_t g_next = b < a.list_cap ? a.list[b] : 0u;
+  uint32_t g_next = b < a.list_cap ? a.cand_list[b] : 0u;
+  const uint32_t ncand = *a.cand_count;
+  const uint32_t count = ncand + *a.list_count;
   for (; b < count; b += stride) {
-    const uint32_t g = g_next;
-    g_next = b + stride < a.list_cap ? a.list[b + stride] : 0u;
+    const uint32_t g = b < ncand ? g_next : a.list[b - ncand];
+    g_next = b + stride < a.list_cap ? a.cand_list[b + stride] : 0u;
     const int sid = one_scene ? 0 : a.block_scene[g];
     const SceneDev sc = a.scenes[sid];
     const int local = (int)g - sc.block_base;
@@ -934,6 +929,7 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       s[1] += q.x;
       s[2] += q.y;
       s[3] += q.z;
+      s[7] += m > 0.f ? 1.0 : 0.0;  // nodes with mass
     }
     if (m > 0.f) {
       const float im = 1.f / m;
@@ -981,7 +977,7 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
   }
   if (with_sums) {
 #pragma unroll
-    for (int c = 0; c < 7; ++c) {
+    for (int c = 0; c < 8; ++c) {
       const double r = wave_sum(s[c]);
       if ((t & 63) == 0) red[wv][c] = r;
     }
@@ -990,7 +986,7 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
   if (t == 0 && with_sums) {
     const int step = (int)a.counters[2] - len + k;
     if (step >= 0 && step < a.max_steps) {
-      for (int c = 0; c < 7; ++c) {
+      for (int c = 0; c < 8; ++c) {
         double r = 0;
         for (int w = 0; w < ENG_THREADS / 64; ++w) r += red[w][c];
         atomicAdd(a.sums + (size_t)step * 8 + c, r);
@@ -1078,6 +1074,32 @@ __global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
     }
   }
   if (i == n - 1) *nchunks = idx[i] + flags[i];
+}
+
+// Candidate blocks of the grid update for the new binning: first the flags
+// of the previous list are cleared, then (after the count is reset) every
+// chunk marks the 27 blocks around its own; each block is appended once.
+__global__ void cand_clear_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
+                                  uint32_t* __restrict__ flag) {
+  const uint32_t n = *count;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) flag[list[i]] = 0u;
+}
+
+__global__ void cand_build_kernel(EngineArgs a, uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                                  uint32_t* __restrict__ flag) {
+  const uint32_t n = *a.nchunks * 27u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int4 ch = a.chunks[i / 27u];
+    const int nbh = (int)(i % 27u);
+    const SceneDev sc = a.scenes[ch.w];
+    const int local = ch.z - sc.block_base;
+    const int nb = sc.nb;
+    const int bi = local / (nb * nb) - 1 + nbh / 9, bj = (local / nb) % nb - 1 + (nbh / 3) % 3,
+              bk = local % nb - 1 + nbh % 3;
+    if (bi < 0 || bj < 0 || bk < 0 || bi >= nb || bj >= nb || bk >= nb) continue;
+    const uint32_t g = (uint32_t)(sc.block_base + (bi * nb + bj) * nb + bk);
+    if (flag[g] == 0u && atomicOr(flag + g, 1u) == 0u) list[atomicAdd(count, 1u)] = g;
+  }
 }
 
 __global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
@@ -1320,6 +1342,9 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   if ((rc = alloc((void**)&d_cidx, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_part, sizeof(uint32_t) * (n / 4096 + 4)))) return rc;
   if ((rc = alloc((void**)&d_hard, sizeof(uint32_t) * n))) return rc;
+  if ((rc = alloc((void**)&d_cand_list, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_cand_flag, sizeof(uint32_t) * blocks))) return rc;
+  HIPCHK(hipMemset(d_cand_flag, 0, sizeof(uint32_t) * blocks));
   ws_bytes = radix_workspace_bytes(n);
   if ((rc = alloc(&d_ws, ws_bytes))) return rc;
   HIPCHK(hipMemset(d_acc, 0, sizeof(float4) * 64 * blocks));
@@ -1343,6 +1368,9 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.acc = d_acc;
   a.vel = d_vel;
   a.flags = d_flags;
+  a.cand_list = d_cand_list;
+  a.cand_count = d_counters + 4;
+  a.cand_flag = d_cand_flag;
   a.list = d_list + (size_t)sel * (size_t)total_blocks;
   a.list_count = d_counters + sel;
   a.other_count = d_counters + (sel ^ 1);
@@ -1411,6 +1439,8 @@ int Engine::download(mpm_particles* p) {
 void Engine::enqueue_bin(hipStream_t s) {
   if (n <= 0) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_nchunks, (int64_t)1, 0u);
+    hipLaunchKernelGGL(cand_clear_kernel, dim3(num_cus), dim3(256), 0, s, d_cand_list, d_counters + 4, d_cand_flag);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_counters + 4, (int64_t)1, 0u);
     return;
   }
   EngineArgs a = args(nullptr, 0, 0);
@@ -1430,6 +1460,10 @@ void Engine::enqueue_bin(hipStream_t s) {
   hipLaunchKernelGGL(chunk_write_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_cflags, d_cidx,
                      d_bend, d_block_scene, chunk, d_chunks, d_nchunks, d_bfirst, d_bcount);
   d_sorted_keys = ks;
+  hipLaunchKernelGGL(cand_clear_kernel, dim3(num_cus), dim3(256), 0, s, d_cand_list, d_counters + 4, d_cand_flag);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_counters + 4, (int64_t)1, 0u);
+  hipLaunchKernelGGL(cand_build_kernel, dim3(num_cus * 2), dim3(256), 0, s, args(nullptr, 0, 0), d_cand_list,
+                     d_counters + 4, d_cand_flag);
 }
 
 void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
@@ -1622,9 +1656,9 @@ void Engine::drop_graphs() {
 // Blocks the last substep's P2G touched (the grid update's work list).
 int Engine::active_blocks(int64_t* out) {
   HIPCHK(hipStreamSynchronize(own_stream));
-  uint32_t v = 0;
-  HIPCHK(hipMemcpy(&v, d_counters + last_list, sizeof(v), hipMemcpyDeviceToHost));
-  *out = (int64_t)v;
+  uint32_t v[5] = {0, 0, 0, 0, 0};
+  HIPCHK(hipMemcpy(v, d_counters, sizeof(v), hipMemcpyDeviceToHost));
+  *out = (int64_t)v[4] + (int64_t)v[last_list];
   return MPM_OK;
 }
 

@@ -64,11 +64,19 @@ struct EngineArgs {
   float4* acc;
   float4* vel;
   uint32_t* flags;
-  uint32_t* list;        // active blocks appended by this substep's P2G
+  // The grid update's work list is the candidate blocks -- the 27
+  // neighbours of every block that holds particles, built at each rebin:
+  // every node a chunk's LDS arena reaches lies in them -- followed by the
+  // blocks marked during the substep by the paths that reach past the arena
+  // (global-atomic scatter of drifted particles, the deferred FP64 pass).
+  const uint32_t* cand_list;
+  const uint32_t* cand_count;
+  const uint32_t* cand_flag;  // nonzero: the block is a candidate this segment
+  uint32_t* list;        // blocks marked during this substep (not candidates)
   uint32_t* list_count;  // their number
   uint32_t* other_count; // the other list's count (cleared by the grid update)
-  uint32_t* counters;  // [0], [1] active-block list counts, [2] sums step base,
-                       // [3] deferred particles
+  uint32_t* counters;  // [0], [1] marked-block list counts, [2] sums step base,
+                       // [3] deferred particles, [4] candidate blocks
   uint32_t* hard_list;   // deferred particles: index << 2 | plasticity pending | no P2G
   uint32_t* hard_count;  // == counters + 3
   const int4* chunks;  // start, count, global block, scene (one per occupied block)
@@ -126,13 +134,20 @@ __device__ __forceinline__ bool locate3(const float* xp, const SceneDev& sc, int
   return ok;
 }
 
+// Block g must be in this substep's grid update: candidates always are.
 __device__ __forceinline__ void mark_active(const EngineArgs& a, uint32_t g) {
+  if (a.cand_flag[g]) return;
   if (__hip_atomic_load(a.flags + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
     if (atomicOr(a.flags + g, 1u) == 0u) {
       const uint32_t idx = atomicAdd(a.list_count, 1u);
       a.list[idx] = g;
     }
   }
+}
+
+// Entry b of the grid update's work list (candidates, then marked blocks).
+__device__ __forceinline__ uint32_t active_block(const EngineArgs& a, uint32_t b, uint32_t ncand) {
+  return b < ncand ? a.cand_list[b] : a.list[b - ncand];
 }
 
 }  // namespace mpm

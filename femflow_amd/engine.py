@@ -138,7 +138,7 @@ class Engine:
     def run(self, nsteps: int, sums: Optional[torch.Tensor] = None, check_oob: bool = True):
         """Advance every scene by ``nsteps`` substeps.  ``sums``: optional FP64
         CUDA tensor (nsteps, 8) receiving per-substep grid conservation sums
-        [mass, momentum after P2G (3), momentum after the grid update (3), 0]."""
+        [mass, momentum after P2G (3), momentum after the grid update (3), nodes with mass]."""
         self._sync_stream()
         if sums is not None:
             self._check_tensor(sums, "sums", torch.float64)

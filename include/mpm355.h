@@ -173,7 +173,7 @@ int32_t mpm_state_upload(mpm_ctx* ctx, const mpm_particles* p, const int32_t* sc
 int32_t mpm_state_download(mpm_ctx* ctx, mpm_particles* p);
 /* Advance every scene by nsteps substeps.  If sums != NULL (device double,
  * nsteps * 8 entries) each substep writes FP64 [grid mass, grid momentum x3
- * after P2G, grid momentum x3 after the update, particle-count] (see docs). */
+ * after P2G, grid momentum x3 after the update, number of nodes with mass]. */
 int32_t mpm_run(mpm_ctx* ctx, int32_t nsteps, double* sums);
 /* Like mpm_run (no sums), also writing each substep's positions in the
  * caller's order to frames (device float, nsteps * n * 3): the per-step frame

@@ -18,6 +18,7 @@
  * singular values, the clamped F in the U S V^T gauge) are compared.
  */
 #include <math.h>
+#include <stdlib.h>
 #include <stdint.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -184,56 +185,115 @@ static inline void weights1(double fx, double *w) {
   w[2] = 0.5 * (fx - 0.5) * (fx - 0.5);
 }
 
-/* three_d/p2g.py:14-80.  grid_v: (res+1)^3 x 3, grid_m: (res+1)^3; accumulated. */
+/* One particle of three_d/p2g.py:49-80, restricted to the nodes whose first
+ * index lies in [ilo, ihi): weights, stress + affine, then the 27 nodes in
+ * lexicographic (i, j, k) order.  Returns 1 if its stencil leaves the grid
+ * (utils.py:138-150), adding nothing. */
+static int p2g_particle3d(const oracle_particles *P, const oracle_params *prm, int64_t p, int64_t ilo,
+                          int64_t ihi, double *grid_v, double *grid_m) {
+  const int64_t r1 = prm->res + 1;
+  const double *xp = P->x + 3 * p;
+  int64_t base[3];
+  double fx[3], w[3][3];
+  int bad = 0;
+  for (int d = 0; d < 3; ++d) {
+    double sd = xp[d] * prm->inv_dx;
+    base[d] = (int64_t)(sd - 0.5);
+    fx[d] = sd - (double)base[d];
+    weights1(fx[d], w[d]);
+    if (base[d] < 0 || base[d] + 2 >= r1) bad = 1;
+  }
+  if (bad) return 1;
+  double mu, lam;
+  lame(P->kind[p], P->h[p], P->mu0[p], P->lam0[p], P->Jp[p], &mu, &lam);
+  double A[9];
+  affine3(P->F + 9 * p, P->C + 9 * p, P->mass[p], mu, lam, P->kind[p], prm->inv_dx, prm->dt,
+          prm->volume, prm->textbook, A);
+  const double m = P->mass[p];
+  const double *vp = P->v + 3 * p;
+  for (int i = 0; i < 3; ++i) {
+    if (base[0] + i < ilo || base[0] + i >= ihi) continue;
+    for (int j = 0; j < 3; ++j)
+      for (int k = 0; k < 3; ++k) {
+        double dpos[3] = {(i - fx[0]) * prm->dx, (j - fx[1]) * prm->dx, (k - fx[2]) * prm->dx};
+        double weight = w[0][i] * w[1][j] * w[2][k];
+        int64_t node = ((base[0] + i) * r1 + (base[1] + j)) * r1 + (base[2] + k);
+        for (int d = 0; d < 3; ++d) {
+          double ad = A[3 * d] * dpos[0] + A[3 * d + 1] * dpos[1] + A[3 * d + 2] * dpos[2];
+          grid_v[3 * node + d] += weight * (vp[d] * m + ad);
+        }
+        grid_m[node] += weight * m;
+      }
+  }
+  return 0;
+}
+
+/* three_d/p2g.py:14-80.  grid_v: (res+1)^3 x 3, grid_m: (res+1)^3; accumulated.
+ * With several threads, each owns a slab of node planes (first index in
+ * [ilo, ihi), slabs balanced by the particles that reach them) and walks
+ * every particle in list order, adding only to its own nodes.  Each node
+ * therefore receives its contributions in the reference's order (particle
+ * major, then i, j, k), and the result is bit-identical to one thread's
+ * whatever the thread count: a deterministic FP64 reference (finite
+ * differences of two rollouts see no summation-order noise).  Particles
+ * whose stencil spans two slabs are evaluated by both threads. */
 int oracle_p2g3d(const oracle_particles *P, const oracle_params *prm, double *grid_v,
                  double *grid_m, int nthreads) {
   const int64_t r1 = prm->res + 1;
   int oob = 0;
+  int nt = 1;
 #ifdef _OPENMP
-  if (nthreads > 0) omp_set_num_threads(nthreads);
-#pragma omp parallel for schedule(static) reduction(| : oob) if (nthreads != 1)
+  nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #endif
-  for (int64_t p = 0; p < P->n; ++p) {
-    const double *xp = P->x + 3 * p;
-    int64_t base[3];
-    double fx[3], w[3][3];
-    int bad = 0;
-    for (int d = 0; d < 3; ++d) {
-      double sd = xp[d] * prm->inv_dx;
-      base[d] = (int64_t)(sd - 0.5);
-      fx[d] = sd - (double)base[d];
-      weights1(fx[d], w[d]);
-      if (base[d] < 0 || base[d] + 2 >= r1) bad = 1;
-    }
-    if (bad) { oob |= 1; continue; }
-    double mu, lam;
-    lame(P->kind[p], P->h[p], P->mu0[p], P->lam0[p], P->Jp[p], &mu, &lam);
-    double A[9];
-    affine3(P->F + 9 * p, P->C + 9 * p, P->mass[p], mu, lam, P->kind[p], prm->inv_dx, prm->dt,
-            prm->volume, prm->textbook, A);
-    const double m = P->mass[p];
-    const double *vp = P->v + 3 * p;
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j)
-        for (int k = 0; k < 3; ++k) {
-          double dpos[3] = {(i - fx[0]) * prm->dx, (j - fx[1]) * prm->dx, (k - fx[2]) * prm->dx};
-          double weight = w[0][i] * w[1][j] * w[2][k];
-          int64_t node = ((base[0] + i) * r1 + (base[1] + j)) * r1 + (base[2] + k);
-          for (int d = 0; d < 3; ++d) {
-            double ad = A[3 * d] * dpos[0] + A[3 * d + 1] * dpos[1] + A[3 * d + 2] * dpos[2];
-            double c = weight * (vp[d] * m + ad);
-#ifdef _OPENMP
-#pragma omp atomic
-#endif
-            grid_v[3 * node + d] += c;
-          }
-          double cm = weight * m;
-#ifdef _OPENMP
-#pragma omp atomic
-#endif
-          grid_m[node] += cm;
-        }
+  if (nt <= 1 || P->n == 0) {
+    for (int64_t p = 0; p < P->n; ++p) oob |= p2g_particle3d(P, prm, p, 0, r1, grid_v, grid_m);
+    return oob ? 1 : 0;
   }
+  /* work per node plane i: particles whose stencil base is i-2, i-1 or i */
+  int64_t *work = (int64_t *)calloc((size_t)r1 + 1, sizeof(int64_t));
+  int64_t *bnd = (int64_t *)malloc(sizeof(int64_t) * ((size_t)nt + 1));
+  if (!work || !bnd) {
+    free(work);
+    free(bnd);
+    for (int64_t p = 0; p < P->n; ++p) oob |= p2g_particle3d(P, prm, p, 0, r1, grid_v, grid_m);
+    return oob ? 1 : 0;
+  }
+  for (int64_t p = 0; p < P->n; ++p) {
+    int64_t b = (int64_t)(P->x[3 * p] * prm->inv_dx - 0.5);
+    b = b < 0 ? 0 : (b > r1 - 1 ? r1 - 1 : b);
+    for (int i = 0; i < 3 && b + i < r1; ++i) work[b + i] += 1;
+  }
+  int64_t total = 0;
+  for (int64_t i = 0; i < r1; ++i) total += work[i];
+  bnd[0] = 0;
+  {
+    int64_t acc = 0, i = 0;
+    for (int t = 1; t < nt; ++t) {
+      const int64_t goal = (total * t + nt - 1) / nt;
+      while (i < r1 && acc < goal) acc += work[i++];
+      bnd[t] = i;
+    }
+    bnd[nt] = r1;
+  }
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nt) reduction(| : oob)
+#endif
+  {
+    int tid = 0;
+#ifdef _OPENMP
+    tid = omp_get_thread_num();
+#endif
+    const int64_t ilo = bnd[tid], ihi = bnd[tid + 1];
+    if (ilo < ihi) {
+      for (int64_t p = 0; p < P->n; ++p) {
+        const int64_t b = (int64_t)(P->x[3 * p] * prm->inv_dx - 0.5);
+        if (b + 2 < ilo || b >= ihi) continue;
+        oob |= p2g_particle3d(P, prm, p, ilo, ihi, grid_v, grid_m);
+      }
+    }
+  }
+  free(work);
+  free(bnd);
   return oob ? 1 : 0;
 }
 

@@ -204,6 +204,32 @@ def test_rollout_gradient_matches_fp64_reference(gpu, budget):
         assert e < 5e-3, name
 
 
+def _fp64_gpu_loss(gpu, x0, v0, mu0, lam0, mass, vol, res, dt, steps, sub, target):
+    """FP64 dense-grid forward on the GPU (the library's FP64 phase kernels)."""
+    from femflow_amd import _lib
+    from femflow_amd.solvers.mpm import _phase
+
+    n = x0.shape[0]
+    f64 = lambda a: torch.as_tensor(a, dtype=torch.float64, device=gpu).clone().contiguous()  # noqa: E731
+    x, v = f64(x0), f64(v0)
+    F = torch.eye(3, dtype=torch.float64, device=gpu).repeat(n, 1, 1)
+    C = torch.zeros((n, 3, 3), dtype=torch.float64, device=gpu)
+    Jp = torch.ones(n, dtype=torch.float64, device=gpu)
+    sp = _phase.StagedParticles(x, v, F, C, Jp, f64(mass), f64(mu0), f64(lam0))
+    prm = _phase.make_params(3, res, float(res), 1.0 / res, dt, vol, -9.8)
+    r1 = res + 1
+    gv = torch.zeros((r1, r1, r1, 3), dtype=torch.float64, device=gpu)
+    gm = torch.zeros((r1, r1, r1, 1), dtype=torch.float64, device=gpu)
+    for _ in range(steps):
+        gv.zero_()
+        gm.zero_()
+        assert _phase.run_p2g(sp, prm, _lib.MPM_ELASTIC, 1.0, gv, gm) == 0
+        _phase.run_grid_op(prm, gv, gm)
+        assert _phase.run_g2p(sp, prm, _lib.MPM_ELASTIC, gv) == 0
+    c = x[torch.as_tensor(sub, device=gpu)].mean(0)
+    return float(((c - torch.as_tensor(target, dtype=torch.float64, device=gpu)) ** 2).sum())
+
+
 def full_size_adjoint_case(seed=0):
     """The full-size gradient workload: 2^17 particles uniform in the cube
     [0.4, 0.6]^3 on a 64^3 grid, E ~ U[1e3, 1e4] per particle (nu = 0.2),
@@ -223,6 +249,12 @@ def full_size_adjoint_case(seed=0):
     vol = 0.2 ** 3 / n
     return dict(n=n, res=res, dt=dt, steps=steps, x0=x0, v0=v0, E=E, sub=sub, target=target, vol=vol,
                 mass=np.full(n, vol), rng=rng)
+
+
+# bars on directional derivatives against the FP64 C restatement, about ten
+# times the agreement measured on one MI355X (v0: 1.6e-5 for the upper-half
+# loss, 9.5e-6 for the whole-body one; E: 1.1e-4 uniform, 5.7e-5 random)
+V0_BAR, E_BAR = 1e-4, 1e-3
 
 
 def _c_oracle_losses(x0, v0, E, mass, vol, res, dt, steps, sub, tgt_sub, tgt_all):
@@ -271,8 +303,10 @@ def test_rollout_full_size_matches_central_differences(gpu):
     mu0, lam0 = lame_from_E(Et, 0.2)
     xT, _ = rollout(f32(c["x0"]), v0t, mu0, lam0, f32(c["mass"]), cfg)
     L = ((xT[sub].mean(0) - f32(c["target"])) ** 2).sum()
-    L_all = ((xT.mean(0) - f32(tgt_all)) ** 2).sum()
-    gv, gE = torch.autograd.grad(L, (v0t, Et), retain_graph=True)
+    gv, gE = torch.autograd.grad(L, (v0t, Et))
+    # the backward consumes its forward's record: one forward per loss
+    xT2, _ = rollout(f32(c["x0"]), v0t, *lame_from_E(Et, 0.2), f32(c["mass"]), cfg)
+    L_all = ((xT2.mean(0) - f32(tgt_all)) ** 2).sum()
     (gv_all,) = torch.autograd.grad(L_all, (v0t,))
     torch.cuda.synchronize()
     print(f"full-size rollout forward+backward: {time.perf_counter() - t0:.2f} s, loss {float(L):.6e}", flush=True)
@@ -286,16 +320,26 @@ def test_rollout_full_size_matches_central_differences(gpu):
     def losses(v0, E):
         return _c_oracle_losses(x0_, v0, E, m_, vol, res, dt, steps, c["sub"], tgt, tga)
 
+    # an O(1) direction per entry (a unit 2-norm over 393k entries would make
+    # each entry's step ~1e-7 and the difference of losses ~1e-10 of L, at
+    # the resolution of two FP64 rollouts)
     d = rng.standard_normal((n, 3))
-    d /= np.linalg.norm(d)
     lp, lm = losses(v0_ + h * d, E_), losses(v0_ - h * d, E_)
-    for name, g, k in (("upper half", gv, 0), ("whole body", gv_all, 1)):
+    rows = []
+    for name, g, k in (("v0, upper half", gv, 0), ("v0, whole body", gv_all, 1)):
         fd = (lp[k] - lm[k]) / (2 * h)
-        ad = float((g.double().cpu().numpy() * d).sum())
-        print(f"dL/dv0 . d [{name}]: adjoint {ad:.6e} fd {fd:.6e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
-        assert abs(ad - fd) <= 1e-4 * abs(fd), name
+        rows.append((name, float((g.double().cpu().numpy() * d).sum()), fd, V0_BAR))
+    # the library's own FP64 forward as a second reference (same FP32 inputs)
+    mu, lam = lame_from_E(torch.as_tensor(E_), 0.2)
+    fd_gpu = (_fp64_gpu_loss(gpu, x0_, v0_ + h * d, mu, lam, m_, vol, res, dt, steps, c["sub"], tgt)
+              - _fp64_gpu_loss(gpu, x0_, v0_ - h * d, mu, lam, m_, vol, res, dt, steps, c["sub"], tgt)) / (2 * h)
+    print(f"dL/dv0 . d [upper half]: FP64 GPU phase-kernel forward {fd_gpu:.9e}, C restatement {rows[0][2]:.9e}",
+          flush=True)
     for name, dE in (("uniform", E_.copy()), ("random", E_ * rng.standard_normal(n))):
         fd = (losses(v0_, E_ + h * dE)[0] - losses(v0_, E_ - h * dE)[0]) / (2 * h)
-        ad = float((gE.double().cpu().numpy() * dE).sum())
-        print(f"dL/dE . d [{name}]: adjoint {ad:.6e} fd {fd:.6e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
-        assert abs(ad - fd) <= 2e-3 * abs(fd), name
+        rows.append((f"E, {name}", float((gE.double().cpu().numpy() * dE).sum()), fd, E_BAR))
+    for name, ad, fd, bar in rows:
+        print(f"dL/d{name}: adjoint {ad:.9e} fd {fd:.9e} rel {abs(ad - fd) / abs(fd):.2e} (bar {bar:.0e})",
+              flush=True)
+    for name, ad, fd, bar in rows:
+        assert abs(ad - fd) <= bar * abs(fd), name

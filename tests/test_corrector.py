@@ -252,7 +252,9 @@ def test_trained_corrector_beats_uncorrected_big_step(gpu):
         ref = eng.positions().clone()
         eng.close()
         for tag, nn in (("plain", None), ("corr", net)):
-            out = CT.CorrectedRunner(cfg, nn, s["volume"], gpu).run(s, 1, check=False)
+            runner = CT.CorrectedRunner(cfg, nn, s["volume"], gpu)
+            out = runner.run(s, 1, check=False)
+            runner.close()
             disp = (ref - s["x"]).norm()
             errs[tag].append(float((out["x"] - ref).norm() / disp))
     gen.close()
@@ -279,3 +281,38 @@ def test_shipped_corrector_one_step_gain(gpu):
     gain = CT.one_step_gain(CT.CorrectorConfig(), net, gpu)
     print("shipped corrector, one big step:", gain, flush=True)
     assert gain["with"] < 0.5 * gain["without"]
+
+
+@pytest.mark.gpu
+def test_judge_falls_back_to_plain_substeps(gpu):
+    """The judge keeps a corrected window only within its error tolerance:
+    with a zero tolerance every judged window is replaced by the plain
+    substeps' result, so the judged rollout equals the plain rollout; with an
+    infinite tolerance it keeps the big steps (up to the health check)."""
+    import torch
+
+    from femflow_amd import corrector_train as CT
+
+    cfg = CT.CorrectorConfig(n=1 << 15, res=64)
+    s = CT.scene_tensors(cfg, 0, gpu)
+    n_big = 4
+    plain = CT.make_engine(cfg, s["volume"], cfg.dt, cfg.gravity, gpu, s["x"].shape[0], sort_every=48)
+    CT._upload(plain, s)
+    plain.run(n_big * cfg.factor)
+    ref = CT._download(plain, s)
+    plain.close()
+    strict = CT.CorrectedRunner(cfg, None, s["volume"], gpu, window=2, judge_every=1, tol=0.0)
+    out = strict.run(s, n_big)
+    st = dict(strict.stats)
+    strict.close()
+    torch.cuda.synchronize()
+    assert st["judged"] + st["unhealthy"] == 2 and st["big_steps_kept"] == 0, st
+    assert st["plain_substeps"] == n_big * cfg.factor
+    assert CT.rel_position_error(out["x"], ref["x"]) < 1e-6
+    loose = CT.CorrectedRunner(cfg, None, s["volume"], gpu, window=2, judge_every=1, tol=float("inf"))
+    out2 = loose.run(s, n_big)
+    st2 = dict(loose.stats)
+    loose.close()
+    assert st2["fallbacks"] == 0 and st2["big_steps_kept"] + 2 * st2["unhealthy"] == n_big, st2
+    print("judge stats strict", st, "loose", st2, "loose error", CT.rel_position_error(out2["x"], ref["x"]),
+          flush=True)

@@ -12,7 +12,10 @@
 The per-substep bar (1e-5 against FP64 from identical inputs) is checked in
 test_gpu_parity.py; here FP32 rounding accumulates over the rollout and the
 bounds below are the drift the rollout is allowed (about 3x what was
-measured; the tables are printed).
+measured; the tables are printed).  Measured on one MI355X: 2D FP64 after
+200 substeps <= 1.4e-12; 2D FP32 x 1.3e-7, v 1.9e-6, F 5.6e-8; 3D FP32 after
+1000 substeps x 3.4e-5, v 6.5e-5, F 2.3e-6, growing about linearly from
+3.3e-6 / 9.6e-6 / 6.9e-8 after 100.
 """
 import os
 
@@ -29,8 +32,8 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 # FP32 drift bounds after the full rollouts (relative, tests/_parity.py norms)
-C1_FP32_BOUND = {"x": 1e-5, "v": 1e-3, "F": 1e-4}
-C2_FP32_BOUND = {"x": 1e-4, "v": 1e-2, "F": 1e-3}
+C1_FP32_BOUND = {"x": 5e-7, "v": 6e-6, "F": 2e-7}
+C2_FP32_BOUND = {"x": 1e-4, "v": 2e-4, "F": 1e-5}
 
 
 def _threads():

@@ -406,3 +406,14 @@ def test_config1_full_rollout_fixture():
     b = CO.run(st, pr, 200).to_state()
     errs = field_errors(b, {k: g[f"s200_{k}"] for k in ("x", "v", "F", "C", "Jp")}, pr["inv_dx"])
     assert max(errs.values()) <= 1e-9, errs
+
+
+def test_c_restatement_threads_bit_identical():
+    """The C restatement's parallel P2G owns node slabs and walks particles in
+    list order, so any thread count reproduces the one-thread result bit for
+    bit (a deterministic FP64 reference for finite differences)."""
+    st, pr = O.scene_3d_cubes(n=20000, res=48)
+    runs = [CO.run(st, pr, 4, nthreads=nt).arrays for nt in (1, 2, 5)]
+    for other in runs[1:]:
+        for k in ("x", "v", "F", "C", "Jp"):
+            assert np.array_equal(runs[0][k], other[k]), k
