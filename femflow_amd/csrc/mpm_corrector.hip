@@ -361,11 +361,16 @@ int Engine::corrector_sample(uint16_t* X, uint32_t* nodes, float* vel, int64_t m
     set_error("mpm_corrector_sample: stream setup failed");
     return MPM_ERR_HIP;
   }
-  begin_segment(s, 1, 0);
+  begin_segment(s, 1, 0, 1);
   enqueue_bin(s);
   since_bin = 0;  // freshly binned: the next run continues from this order
   const EngineArgs a = args(nullptr, 0, 0);
-  launch_transfer(0, s, a);
+  {
+    EngineArgs a0 = a;
+    a0.perm = pending_perm;  // this P2G moves the particles into binned order
+    pending_perm = nullptr;
+    launch_transfer(0, s, a0);
+  }
   launch_cellf(s, a);
   launch_grid(s, a, 0, 1, 0);
   fill_u32(s, d_corr_count, 1, 0u);

@@ -85,9 +85,12 @@ struct Engine {
   uint32_t* d_counters = nullptr;
   unsigned long long *d_oob = nullptr, *d_fallback = nullptr;
   int4* d_chunks = nullptr;
-  uint32_t *d_nchunks = nullptr, *d_keys = nullptr, *d_vals = nullptr, *d_cflags = nullptr,
+  uint32_t *d_nchunks = nullptr, *d_keys = nullptr, *d_vals = nullptr,
            *d_cidx = nullptr, *d_part = nullptr;
   uint32_t* d_sorted_keys = nullptr;
+  // sort permutation of the last rebin, consumed by the next P2G (which
+  // moves the particles into binned order); null once consumed
+  const uint32_t* pending_perm = nullptr;
   uint32_t* d_hard = nullptr;
   uint32_t *d_cand_list = nullptr, *d_cand_flag = nullptr;
   float4* d_stage = nullptr;
@@ -125,7 +128,7 @@ struct Engine {
   int positions(float* x);
   void launch_transfer(int mode, hipStream_t s, const EngineArgs& a);
   void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k);
-  void begin_segment(hipStream_t s, int len, int with_sums);
+  void begin_segment(hipStream_t s, int len, int with_sums, int bin);
   void fill_u32(hipStream_t s, uint32_t* p, int64_t count, uint32_t v);
   void launch_cellf(hipStream_t s, const EngineArgs& a);
   // corrector after a grid update; ex != null: write features instead (sampling)

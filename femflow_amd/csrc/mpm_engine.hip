@@ -485,17 +485,50 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
       if constexpr (MODE != 2) pb_s[qs] = -1;
       const int64_t p = (int64_t)ch.x + q;
       float x[3], F[9], v[3], C[9], Jp, w[3][3], fx[3];
+      float pmass, pmu0, plam0;
       int base[3];
-      loadk<3>(a.s.x, p, x);
-      loadk<9>(a.s.F, p, F);
-      Jp = a.s.Jp[p];
-      int mid = (int)(a.s.attr[p] & 255u);
+      uint32_t attr;
+      if (MODE == 0 && a.perm) {
+        // binned slot p takes the particle of slot perm[p] of the previous
+        // order: every field is copied here, before any early exit
+        const int64_t ps = a.perm[p];
+        loadk<3>(a.src.x, ps, x);
+        loadk<9>(a.src.F, ps, F);
+        loadk<3>(a.src.v, ps, v);
+        loadk<9>(a.src.C, ps, C);
+        Jp = a.src.Jp[ps];
+        attr = a.src.attr[ps];
+        pmass = a.src.mass[ps];
+        pmu0 = a.src.mu0[ps];
+        plam0 = a.src.lam0[ps];
+        const uint32_t ord = a.src.order[ps];
+        storek<3>(a.s.x, p, x);
+        storek<9>(a.s.F, p, F);
+        storek<3>(a.s.v, p, v);
+        storek<9>(a.s.C, p, C);
+        a.s.Jp[p] = Jp;
+        a.s.attr[p] = attr;
+        a.s.mass[p] = pmass;
+        a.s.mu0[p] = pmu0;
+        a.s.lam0[p] = plam0;
+        a.s.order[p] = ord;
+      } else {
+        loadk<3>(a.s.x, p, x);
+        loadk<9>(a.s.F, p, F);
+        Jp = a.s.Jp[p];
+        attr = a.s.attr[p];
+        pmass = a.s.mass[p];
+        pmu0 = a.s.mu0[p];
+        plam0 = a.s.lam0[p];
+        if constexpr (MODE == 0) {
+          loadk<3>(a.s.v, p, v);
+          loadk<9>(a.s.C, p, C);
+        }
+      }
+      int mid = (int)(attr & 255u);
       mid = mid < a.nmat ? mid : a.nmat - 1;
       const MatParams mp = a.mats[mid];
-      if constexpr (MODE == 0) {
-        loadk<3>(a.s.v, p, v);
-        loadk<9>(a.s.C, p, C);
-      } else {
+      if constexpr (MODE != 0) {
         if (!locate3(x, sc, base, fx)) {
           atomicAdd(&wg_oob, 1u);
           continue;
@@ -526,7 +559,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
         P2GTerms tm;
-        if (!p2g_terms(sc, mp, a.s.mass[p], a.s.mu0[p], a.s.lam0[p], Jp, F, C, v, fx, tm)) {
+        if (!p2g_terms(sc, mp, pmass, pmu0, plam0, Jp, F, C, v, fx, tm)) {
           defer(a, p, 0u);
           continue;
         }
@@ -788,14 +821,19 @@ __global__ __launch_bounds__(ENG_THREADS) void deferred_kernel(EngineArgs a) {
   if (done_here) atomicAdd(a.fallback, (unsigned long long)done_here);
 }
 
-// Start of a segment: clear both active-block list counters and the deferred
-// list, and advance the per-substep sums index by the segment length.
-__global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums) {
+// Start of a segment: clear both marked-block list counters and the deferred
+// list, advance the per-substep sums index by the segment length, and, when
+// the segment rebins, start a new candidate list (count 0, next generation).
+__global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums, int bin) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     counters[0] = 0u;
     counters[1] = 0u;
     counters[3] = 0u;
     if (with_sums) counters[2] += (uint32_t)len;
+    if (bin) {
+      counters[4] = 0u;
+      counters[5] += 1u;
+    }
   }
 }
 
@@ -1016,29 +1054,6 @@ __global__ void engine_keys_kernel(EngineArgs a, int64_t n, uint32_t* __restrict
   keys[i] = blk * 64u + (uint32_t)(((b0 & 3) << 4) | ((b1 & 3) << 2) | (b2 & 3));
 }
 
-__global__ void reorder_kernel(EngineState src, EngineState dst, const uint32_t* __restrict__ perm,
-                               int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int64_t s = perm[i];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    dst.x[tix(i, 3, d)] = src.x[tix(s, 3, d)];
-    dst.v[tix(i, 3, d)] = src.v[tix(s, 3, d)];
-  }
-#pragma unroll
-  for (int d = 0; d < 9; ++d) {
-    dst.F[tix(i, 9, d)] = src.F[tix(s, 9, d)];
-    dst.C[tix(i, 9, d)] = src.C[tix(s, 9, d)];
-  }
-  dst.Jp[i] = src.Jp[s];
-  dst.mass[i] = src.mass[s];
-  dst.mu0[i] = src.mu0[s];
-  dst.lam0[i] = src.lam0[s];
-  dst.attr[i] = src.attr[s];
-  dst.order[i] = src.order[s];
-}
-
 __global__ void block_bounds_kernel(const uint32_t* __restrict__ keys, int64_t n,
                                     uint32_t* __restrict__ bstart, uint32_t* __restrict__ bend) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1048,22 +1063,14 @@ __global__ void block_bounds_kernel(const uint32_t* __restrict__ keys, int64_t n
   if (i == n - 1 || (keys[i + 1] >> 6) != blk) bend[blk] = (uint32_t)(i + 1);
 }
 
-__global__ void chunk_flags_kernel(const uint32_t* __restrict__ keys, int64_t n,
-                                   const uint32_t* __restrict__ bstart, int chunk,
-                                   uint32_t* __restrict__ flags) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  flags[i] = (((uint32_t)i - bstart[keys[i] >> 6]) % (uint32_t)chunk) == 0u ? 1u : 0u;
-}
-
 __global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
-                                   const uint32_t* __restrict__ flags, const uint32_t* __restrict__ idx,
+                                   const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ idx,
                                    const uint32_t* __restrict__ bend, const int* __restrict__ block_scene,
-                                   int chunk, int4* __restrict__ chunks, uint32_t* __restrict__ nchunks,
-                                   uint32_t* __restrict__ bfirst, uint32_t* __restrict__ bcount) {
+                                   int chunk, int4* __restrict__ chunks, uint32_t* __restrict__ bfirst,
+                                   uint32_t* __restrict__ bcount) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  if (flags[i]) {
+  if ((((uint32_t)i - bstart[keys[i] >> 6]) % (uint32_t)chunk) == 0u) {
     const uint32_t blk = keys[i] >> 6;
     const uint32_t cnt = bend[blk] - (uint32_t)i;
     chunks[idx[i]] = make_int4((int)i, (int)(cnt < (uint32_t)chunk ? cnt : (uint32_t)chunk), (int)blk,
@@ -1073,20 +1080,17 @@ __global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
       bcount[blk] = (cnt + (uint32_t)chunk - 1u) / (uint32_t)chunk;
     }
   }
-  if (i == n - 1) *nchunks = idx[i] + flags[i];
 }
 
-// Candidate blocks of the grid update for the new binning: first the flags
-// of the previous list are cleared, then (after the count is reset) every
-// chunk marks the 27 blocks around its own; each block is appended once.
-__global__ void cand_clear_kernel(const uint32_t* __restrict__ list, const uint32_t* __restrict__ count,
-                                  uint32_t* __restrict__ flag) {
-  const uint32_t n = *count;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) flag[list[i]] = 0u;
-}
-
+// Candidate blocks of the grid update for the new binning (the segment's
+// start has advanced the binning generation and zeroed the count): every
+// chunk tags the 27 blocks around its own with the generation, and the first
+// tag of a block appends it, so each block is listed once.  A block is a
+// candidate while its tag equals the current generation, so the previous
+// list needs no clearing.
 __global__ void cand_build_kernel(EngineArgs a, uint32_t* __restrict__ list, uint32_t* __restrict__ count,
                                   uint32_t* __restrict__ flag) {
+  const uint32_t gen = *a.cand_gen;
   const uint32_t n = *a.nchunks * 27u;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int4 ch = a.chunks[i / 27u];
@@ -1098,7 +1102,7 @@ __global__ void cand_build_kernel(EngineArgs a, uint32_t* __restrict__ list, uin
               bk = local % nb - 1 + nbh % 3;
     if (bi < 0 || bj < 0 || bk < 0 || bi >= nb || bj >= nb || bk >= nb) continue;
     const uint32_t g = (uint32_t)(sc.block_base + (bi * nb + bj) * nb + bk);
-    if (flag[g] == 0u && atomicOr(flag + g, 1u) == 0u) list[atomicAdd(count, 1u)] = g;
+    if (flag[g] != gen && atomicExch(flag + g, gen) != gen) list[atomicAdd(count, 1u)] = g;
   }
 }
 
@@ -1338,7 +1342,6 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   HIPCHK(hipMemset(d_bcount, 0, sizeof(uint32_t) * blocks));
   if ((rc = alloc((void**)&d_keys, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_vals, sizeof(uint32_t) * n))) return rc;
-  if ((rc = alloc((void**)&d_cflags, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_cidx, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_part, sizeof(uint32_t) * (n / 4096 + 4)))) return rc;
   if ((rc = alloc((void**)&d_hard, sizeof(uint32_t) * n))) return rc;
@@ -1362,6 +1365,8 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
 EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   EngineArgs a{};
   a.s = st[cur];
+  a.src = st[cur ^ 1];
+  a.perm = nullptr;
   a.scenes = d_scenes;
   a.mats = d_mats;
   a.nmat = nmat;
@@ -1371,6 +1376,7 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.cand_list = d_cand_list;
   a.cand_count = d_counters + 4;
   a.cand_flag = d_cand_flag;
+  a.cand_gen = d_counters + 5;
   a.list = d_list + (size_t)sel * (size_t)total_blocks;
   a.list_count = d_counters + sel;
   a.other_count = d_counters + (sel ^ 1);
@@ -1439,8 +1445,6 @@ int Engine::download(mpm_particles* p) {
 void Engine::enqueue_bin(hipStream_t s) {
   if (n <= 0) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_nchunks, (int64_t)1, 0u);
-    hipLaunchKernelGGL(cand_clear_kernel, dim3(num_cus), dim3(256), 0, s, d_cand_list, d_counters + 4, d_cand_flag);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_counters + 4, (int64_t)1, 0u);
     return;
   }
   EngineArgs a = args(nullptr, 0, 0);
@@ -1448,20 +1452,17 @@ void Engine::enqueue_bin(hipStream_t s) {
   uint32_t *ks, *vs;
   const uint32_t max_key = (uint32_t)(total_blocks * 64 - 1);
   radix_sort(d_keys, d_vals, n, max_key, d_ws, true, &ks, &vs, s);
-  const int nxt = cur ^ 1;
-  hipLaunchKernelGGL(reorder_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, st[cur], st[nxt], vs, n);
-  cur = nxt;
+  // the particles move into binned order in the segment's first P2G, which
+  // reads slot perm[p] of the current buffer and writes slot p of the other
+  cur ^= 1;
+  pending_perm = vs;
   hipLaunchKernelGGL(block_bounds_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, d_bend);
-  hipLaunchKernelGGL(chunk_flags_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, chunk,
-                     d_cflags);
-  exclusive_scan(d_cflags, d_cidx, n, d_part, nullptr, s);
+  chunk_index(ks, d_bstart, n, chunk, d_cidx, d_part, d_nchunks, s);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(total_blocks, 256)), dim3(256), 0, s, d_bcount,
                      (int64_t)total_blocks, 0u);
-  hipLaunchKernelGGL(chunk_write_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_cflags, d_cidx,
-                     d_bend, d_block_scene, chunk, d_chunks, d_nchunks, d_bfirst, d_bcount);
+  hipLaunchKernelGGL(chunk_write_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, d_cidx,
+                     d_bend, d_block_scene, chunk, d_chunks, d_bfirst, d_bcount);
   d_sorted_keys = ks;
-  hipLaunchKernelGGL(cand_clear_kernel, dim3(num_cus), dim3(256), 0, s, d_cand_list, d_counters + 4, d_cand_flag);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_counters + 4, (int64_t)1, 0u);
   hipLaunchKernelGGL(cand_build_kernel, dim3(num_cus * 2), dim3(256), 0, s, args(nullptr, 0, 0), d_cand_list,
                      d_counters + 4, d_cand_flag);
 }
@@ -1537,8 +1538,8 @@ void Engine::enqueue_frame(hipStream_t s, float* frames, int64_t step) {
                      frames + (size_t)step * 3 * (size_t)n);
 }
 
-void Engine::begin_segment(hipStream_t s, int len, int with_sums) {
-  hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, with_sums);
+void Engine::begin_segment(hipStream_t s, int len, int with_sums, int bin) {
+  hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, with_sums, bin);
 }
 
 void Engine::fill_u32(hipStream_t s, uint32_t* p, int64_t count, uint32_t v) {
@@ -1555,7 +1556,7 @@ void Engine::enqueue_state(hipStream_t s, const StateSink* sink, int64_t step) {
 void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int max_steps, float* frames,
                              int64_t step0, const StateSink* sink) {
   const int ws = sums ? 1 : 0;
-  hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws);
+  hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws, bin ? 1 : 0);
   if (bin) enqueue_bin(s);
   // substep k appends to active list k % 2; its grid update clears the other
   const size_t rec_nn = (size_t)(scenes_host[0].res + 1) * (scenes_host[0].res + 1) * (scenes_host[0].res + 1);
@@ -1564,7 +1565,12 @@ void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int
     if (sink && sink->grids) a.grid_rec = sink->grids + (size_t)(step0 + k) * 7 * rec_nn;
     return a;
   };
-  launch_transfer(0, s, args(sums, max_steps, 0));
+  {
+    EngineArgs a0 = args(sums, max_steps, 0);
+    a0.perm = bin ? pending_perm : nullptr;
+    pending_perm = nullptr;
+    launch_transfer(0, s, a0);
+  }
   if (corr_on) launch_cellf(s, args(sums, max_steps, 0));
   launch_grid(s, grid_args(0), ws, len, 0);
   if (corr_on) launch_corrector(s, args(sums, max_steps, 0), nullptr);

@@ -58,6 +58,10 @@ struct EngineState {
 
 struct EngineArgs {
   EngineState s;
+  // The first P2G after a rebin also moves the particles into binned order:
+  // slot p of s is read from slot perm[p] of src (perm null: no move).
+  EngineState src;
+  const uint32_t* perm;
   const SceneDev* scenes;
   const MatParams* mats;
   int nmat;
@@ -71,12 +75,14 @@ struct EngineArgs {
   // (global-atomic scatter of drifted particles, the deferred FP64 pass).
   const uint32_t* cand_list;
   const uint32_t* cand_count;
-  const uint32_t* cand_flag;  // nonzero: the block is a candidate this segment
+  const uint32_t* cand_flag;  // == *cand_gen: the block is a candidate of the current binning
+  const uint32_t* cand_gen;   // binning generation (counters[5])
   uint32_t* list;        // blocks marked during this substep (not candidates)
   uint32_t* list_count;  // their number
   uint32_t* other_count; // the other list's count (cleared by the grid update)
   uint32_t* counters;  // [0], [1] marked-block list counts, [2] sums step base,
-                       // [3] deferred particles, [4] candidate blocks
+                       // [3] deferred particles, [4] candidate blocks,
+                       // [5] binning generation
   uint32_t* hard_list;   // deferred particles: index << 2 | plasticity pending | no P2G
   uint32_t* hard_count;  // == counters + 3
   const int4* chunks;  // start, count, global block, scene (one per occupied block)
@@ -136,7 +142,7 @@ __device__ __forceinline__ bool locate3(const float* xp, const SceneDev& sc, int
 
 // Block g must be in this substep's grid update: candidates always are.
 __device__ __forceinline__ void mark_active(const EngineArgs& a, uint32_t g) {
-  if (a.cand_flag[g]) return;
+  if (a.cand_flag[g] == *a.cand_gen) return;
   if (__hip_atomic_load(a.flags + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
     if (atomicOr(a.flags + g, 1u) == 0u) {
       const uint32_t idx = atomicAdd(a.list_count, 1u);

@@ -1,8 +1,8 @@
 // Cell binning: key generation, a stable LSD radix sort on 32-bit keys
 // (8-bit digits; three launches per pass: tile histograms, a per-digit scan
-// over tiles, the stable scatter), a three-phase exclusive scan, and the
-// chunk builder that turns the sorted keys into per-block work items for the
-// transfer kernels.
+// over tiles, the stable scatter), and the chunk numbering (a two-launch
+// scan of chunk-start flags) from which the engine builds its per-block work
+// items for the transfer kernels.
 //
 // Stability: inside a tile each wave owns a contiguous run of keys and walks
 // it in index order; ranks among equal digits come from a 64-lane match mask
@@ -177,53 +177,59 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   return incl - v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void scan_reduce(const uint32_t* __restrict__ in,
-                                                            int64_t n, uint32_t* __restrict__ part) {
+// ------------------------------------------------------- chunk starts ----
+// Sorted key i starts a chunk when its offset from its block's first key is
+// a multiple of `chunk`.  The exclusive scan of those flags numbers the
+// chunks; it runs in two launches: per-tile counts, then each tile adds the
+// counts of the tiles before it (at most a few hundred) and scans itself.
+__device__ __forceinline__ uint32_t chunk_start_flag(const uint32_t* __restrict__ keys,
+                                                     const uint32_t* __restrict__ bstart, int64_t i,
+                                                     int chunk) {
+  return ((((uint32_t)i - bstart[keys[i] >> 6]) % (uint32_t)chunk) == 0u) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void chunk_count_kernel(const uint32_t* __restrict__ keys,
+                                                                   const uint32_t* __restrict__ bstart,
+                                                                   int64_t n, int chunk,
+                                                                   uint32_t* __restrict__ part) {
   __shared__ uint32_t sh[SCAN_THREADS];
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
   uint32_t s = 0;
 #pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; ++i) s += (base + i < n) ? in[base + i] : 0u;
+  for (int i = 0; i < SCAN_ITEMS; ++i) s += (base + i < n) ? chunk_start_flag(keys, bstart, base + i, chunk) : 0u;
   uint32_t tot;
   block_exclusive_scan(s, sh, &tot);
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
-// Single workgroup: exclusive scan of the per-tile partial sums in place.
-__global__ __launch_bounds__(SCAN_THREADS) void scan_partials(uint32_t* __restrict__ part, int64_t np,
-                                                              uint32_t* __restrict__ grand_total) {
+__global__ __launch_bounds__(SCAN_THREADS) void chunk_index_kernel(const uint32_t* __restrict__ keys,
+                                                                   const uint32_t* __restrict__ bstart,
+                                                                   int64_t n, int chunk,
+                                                                   const uint32_t* __restrict__ part,
+                                                                   uint32_t* __restrict__ idx,
+                                                                   uint32_t* __restrict__ nchunks) {
   __shared__ uint32_t sh[SCAN_THREADS];
-  uint32_t carry = 0;
-  for (int64_t b = 0; b < np; b += SCAN_THREADS) {
-    const int64_t i = b + threadIdx.x;
-    const uint32_t v = i < np ? part[i] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(v, sh, &tot);
-    if (i < np) part[i] = carry + ex;
-    carry += tot;
-  }
-  if (threadIdx.x == 0 && grand_total) *grand_total = carry;
-}
-
-__global__ __launch_bounds__(SCAN_THREADS) void scan_downsweep(const uint32_t* __restrict__ in,
-                                                               int64_t n, const uint32_t* __restrict__ part,
-                                                               uint32_t* __restrict__ out) {
-  __shared__ uint32_t sh[SCAN_THREADS];
+  // chunks started in the tiles before this one
+  uint32_t before = 0;
+  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += SCAN_THREADS) before += part[b];
+  uint32_t tot_before;
+  block_exclusive_scan(before, sh, &tot_before);
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-  uint32_t v[SCAN_ITEMS];
+  uint32_t f[SCAN_ITEMS];
   uint32_t s = 0;
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; ++i) {
-    v[i] = (base + i < n) ? in[base + i] : 0u;
-    s += v[i];
+    f[i] = (base + i < n) ? chunk_start_flag(keys, bstart, base + i, chunk) : 0u;
+    s += f[i];
   }
   uint32_t tot;
-  uint32_t run = block_exclusive_scan(s, sh, &tot) + part[blockIdx.x];
+  uint32_t run = block_exclusive_scan(s, sh, &tot) + tot_before;
 #pragma unroll
   for (int i = 0; i < SCAN_ITEMS; ++i) {
-    if (base + i < n) out[base + i] = run;
-    run += v[i];
+    if (base + i < n) idx[base + i] = run;
+    run += f[i];
   }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) *nchunks = run;
 }
 
 // ---------------------------------------------------------------- keys ----
@@ -273,13 +279,14 @@ size_t radix_workspace_bytes(int64_t n) {
   return sizeof(uint32_t) * (4 * (size_t)n + 2 * (size_t)(hist > n ? hist : n) + parts + 256 + 64);
 }
 
-void exclusive_scan(const uint32_t* in, uint32_t* out, int64_t n, uint32_t* part,
-                    uint32_t* total, hipStream_t st) {
+void chunk_index(const uint32_t* keys, const uint32_t* bstart, int64_t n, int chunk, uint32_t* idx,
+                 uint32_t* part, uint32_t* nchunks, hipStream_t st) {
   if (n <= 0) return;
   const int64_t nb = cdiv(n, SCAN_TILE);
-  hipLaunchKernelGGL(scan_reduce, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, in, n, part);
-  hipLaunchKernelGGL(scan_partials, dim3(1), dim3(SCAN_THREADS), 0, st, part, nb, total);
-  hipLaunchKernelGGL(scan_downsweep, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, in, n, part, out);
+  hipLaunchKernelGGL(chunk_count_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, keys, bstart, n, chunk,
+                     part);
+  hipLaunchKernelGGL(chunk_index_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, keys, bstart, n, chunk,
+                     part, idx, nchunks);
 }
 
 int radix_passes(uint32_t max_key) {
