@@ -253,6 +253,16 @@ def baseline_loss(T, w) -> float:
     return float(((T ** 2).sum(1) * w).sum() / w.sum())
 
 
+class RolloutDiverged(RuntimeError):
+    """The plain substeps that judge or replace a window left the grid or
+    produced non-finite positions: the window's start state (a corrected
+    state kept without judging) is beyond what the fallback can recover."""
+
+    def __init__(self, msg: str, big_steps_done: int):
+        super().__init__(msg)
+        self.big_steps_done = big_steps_done
+
+
 class CorrectedRunner:
     """Corrected big steps, judged by the plain substep, which is also the
     fallback.
@@ -266,8 +276,10 @@ class CorrectedRunner:
     |x - x_plain| / |x_plain|, is at most ``tol``; otherwise the plain result
     replaces it (a fallback) and the next window is judged too.  Every
     window, judged or not, must also be healthy -- no stencil left the grid,
-    positions finite, speeds below the grid's velocity clamp -- or it is
-    rerun with plain substeps.  ``judge_every = 1`` judges every window (the
+    positions finite and inside the unit domain, speeds below the big step's
+    one-cell-per-step limit -- or it is rerun with plain substeps.  If the
+    plain substeps themselves fail that check the rollout has diverged
+    beyond recovery and ``run`` raises ``RolloutDiverged``.  ``judge_every = 1`` judges every window (the
     error bound holds window by window, at the cost of the plain substeps);
     larger values trade the bound for throughput.  ``stats`` counts the big
     steps kept, the plain substeps run and the fallbacks."""
@@ -308,11 +320,17 @@ class CorrectedRunner:
     def fallbacks(self):
         return self.stats["fallbacks"] + self.stats["unhealthy"]
 
-    def _healthy(self, eng, cur) -> bool:
-        vmax = 0.9 * (1.0 / self.cfg.res) / self.cfg.dt_big
-        if eng.oob_count():
+    def _healthy(self, eng, cur, dt: float) -> bool:
+        dx = 1.0 / self.cfg.res
+        vmax = 0.9 * dx / dt
+        if eng.oob_count():  # reading resets the engine's counter
             return False
-        return bool(torch.isfinite(cur["x"]).all()) and float(cur["v"].norm(dim=1).max()) < vmax
+        x = cur["x"]
+        if not bool(torch.isfinite(x).all()):
+            return False
+        if float(x.min()) < 0.0 or float(x.max()) > 1.0:
+            return False
+        return float(cur["v"].norm(dim=1).max()) < vmax
 
     def run(self, s: Dict[str, torch.Tensor], n_big: int, check: bool = True) -> Dict[str, torch.Tensor]:
         """n_big big steps of simulated time from state s (caller order);
@@ -336,16 +354,19 @@ class CorrectedRunner:
             cand = _download(big, start)
             st["windows"] += 1
             since_judged += 1
-            ok = self._healthy(big, cand)
+            ok = self._healthy(big, cand, cfg.dt_big)
             if not ok:
                 st["unhealthy"] += 1
             judge = ok and (probation or since_judged >= self.judge_every)
             ref = None
             if judge or not ok:
                 _upload(plain, start)
-                plain.run(k * cfg.factor)
+                plain.run(k * cfg.factor, check_oob=False)
                 st["plain_substeps"] += k * cfg.factor
                 ref = _download(plain, start)
+                if not self._healthy(plain, ref, cfg.dt):
+                    raise RolloutDiverged(f"plain substeps from the state after big step {done} left the "
+                                          f"domain or went non-finite", done)
             if judge:
                 st["judged"] += 1
                 since_judged = 0
@@ -453,18 +474,25 @@ def judged_rollout(cfg: CorrectorConfig, net: Optional[CorrectorMLP], device, n_
     out = {"substeps": n_substeps, "big_steps": n_big, "tol": tol, "plain_seconds": t_plain, "settings": []}
     for window, every in settings:
         runner = CorrectedRunner(cfg, net, s["volume"], device, window=window, judge_every=every, tol=tol)
-        runner.run(s, min(n_big, 2 * window))  # warm-up: engines, graphs
-        runner.reset_stats()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        fin = runner.run(s, n_big)
-        torch.cuda.synchronize()
-        t = time.perf_counter() - t0
-        runner.close()
-        err = rel_position_error(fin["x"], ref)
-        out["settings"].append(dict(window=window, judge_every=every, seconds=t, speedup=t_plain / t,
-                                    rel_position_error=err if math.isfinite(err) else "non-finite",
-                                    within_tol=bool(math.isfinite(err) and err <= tol), **runner.stats))
+        try:
+            runner.run(s, min(n_big, 2 * window))  # warm-up: engines, graphs
+            runner.reset_stats()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fin = runner.run(s, n_big)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            err = rel_position_error(fin["x"], ref)
+            out["settings"].append(dict(window=window, judge_every=every, seconds=t, speedup=t_plain / t,
+                                        rel_position_error=err if math.isfinite(err) else "non-finite",
+                                        within_tol=bool(math.isfinite(err) and err <= tol), **runner.stats))
+        except RolloutDiverged as e:
+            torch.cuda.synchronize()
+            out["settings"].append(dict(window=window, judge_every=every, seconds=None, speedup=None,
+                                        rel_position_error="diverged", diverged_after_big_steps=e.big_steps_done,
+                                        within_tol=False, **runner.stats))
+        finally:
+            runner.close()
     return out
 
 

@@ -93,6 +93,7 @@ struct Engine {
   const uint32_t* pending_perm = nullptr;
   uint32_t* d_hard = nullptr;
   uint32_t *d_cand_list = nullptr, *d_cand_flag = nullptr;
+  uint2* d_cand_src = nullptr;
   float4* d_stage = nullptr;
   // learned corrector
   bool corr_on = false;

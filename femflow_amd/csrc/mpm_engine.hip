@@ -2,6 +2,7 @@
 // G2P(n)+P2G(n+1) transfer, block-sparse grid update, and the rollout driver
 // (eager or replayed as a captured hipGraph).  Reference step semantics are
 // femflow/solvers/mpm/mls_mpm.py:40-79 with three_d/{p2g,grid_op,g2p}.py.
+#include <hip/hip_ext.h>
 #include <string.h>
 
 #include <string>
@@ -866,11 +867,16 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
   const uint32_t stride = gridDim.x * 4u;
   uint32_t b = blockIdx.x * 4u + (uint32_t)wv;
   uint32_t g_next = b < a.list_cap ? a.cand_list[b] : 0u;
+  uint2 src_next = b < a.list_cap && inner < 27 ? a.cand_src[(size_t)b * 32 + inner] : make_uint2(0u, 0u);
   const uint32_t ncand = *a.cand_count;
   const uint32_t count = ncand + *a.list_count;
   for (; b < count; b += stride) {
-    const uint32_t g = b < ncand ? g_next : a.list[b - ncand];
+    const bool cand = b < ncand;
+    const uint32_t g = cand ? g_next : a.list[b - ncand];
+    const uint2 src = src_next;
     g_next = b + stride < a.list_cap ? a.cand_list[b + stride] : 0u;
+    src_next = b + stride < a.list_cap && inner < 27 ? a.cand_src[(size_t)(b + stride) * 32 + inner]
+                                                     : make_uint2(0u, 0u);
     const int sid = one_scene ? 0 : a.block_scene[g];
     const SceneDev sc = a.scenes[sid];
     const int local = (int)g - sc.block_base;
@@ -879,20 +885,17 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
     const int o0 = inner >> 4, o1 = (inner >> 2) & 3, o2 = inner & 3;
     const int i = bi * 4 + o0, j = bj * 4 + o1, k = bk * 4 + o2;
     const size_t addr = (size_t)g * 64 + inner;
-    // lanes 0..26: first chunk and chunk count of the 27 blocks around this one
-    uint32_t my_c0 = 0, my_cn = 0;
-    if (inner < 27) {
-      const int S0 = bi - 1 + inner / 9, S1 = bj - 1 + (inner / 3) % 3, S2 = bk - 1 + inner % 3;
-      if (S0 >= 0 && S0 < nb && S1 >= 0 && S1 < nb && S2 >= 0 && S2 < nb) {
-        const uint32_t S = (uint32_t)(sc.block_base + (S0 * nb + S1) * nb + S2);
-        // both loads in flight together (bfirst is stale where bcount is 0)
-        my_cn = a.bcount[S];
-        my_c0 = a.bfirst[S];
-        my_c0 = my_cn ? my_c0 : 0u;
-        my_cn = my_c0 + my_cn <= a.stage_cap ? my_cn : (my_c0 < a.stage_cap ? a.stage_cap - my_c0 : 0u);
-      }
+    // lanes 0..26: first chunk and chunk count of the 27 blocks around this
+    // one (candidates: from the rebin's table; marked blocks: looked up)
+    uint32_t my_c0 = src.x, my_cn = src.y;
+    if (!cand && inner < 27) {
+      const uint2 nc = neighbour_chunks(a, sc, bi - 1 + inner / 9, bj - 1 + (inner / 3) % 3, bk - 1 + inner % 3);
+      my_c0 = nc.x;
+      my_cn = nc.y;
     }
     float4 q = a.acc[addr];
+    // the accumulator is nonzero only where a global-atomic path wrote
+    const bool acc_used = q.x != 0.f || q.y != 0.f || q.z != 0.f || q.w != 0.f;
     // Staged chunk arenas that cover this node: per axis, node offset o < 3
     // lies in the arenas of blocks b-1 and b, o = 3 in those of b and b+1
     // (an arena spans nodes 4b-1 .. 4b+6).  The first chunk of each of the 8
@@ -992,7 +995,7 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
     for (int pl = 0; pl < a.nplanes; ++pl)
       if (a.planes[pl].scene == sid && plane_blocks(a.planes[pl], i, j, k)) v[0] = v[1] = v[2] = 0.f;
     a.vel[addr] = make_float4(v[0], v[1], v[2], m);
-    a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (acc_used) a.acc[addr] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (a.grid_rec && i <= sc.res && j <= sc.res && k <= sc.res) {
       const size_t r1 = (size_t)sc.res + 1, nn = r1 * r1 * r1;
       const size_t di = ((size_t)i * r1 + (size_t)j) * r1 + (size_t)k;
@@ -1088,6 +1091,27 @@ __global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
 // tag of a block appends it, so each block is listed once.  A block is a
 // candidate while its tag equals the current generation, so the previous
 // list needs no clearing.
+// Per candidate entry (one wave each, lanes 0..26): the staged chunks of
+// its 27 neighbouring blocks.  They are fixed until the next rebin, so the
+// grid update reads them together with the entry instead of looking them
+// up through the block's neighbours.
+__global__ void cand_src_kernel(EngineArgs a, uint2* __restrict__ out) {
+  const uint32_t ncand = *a.cand_count;
+  const int lane = threadIdx.x & 63;
+  const uint32_t waves = gridDim.x * (blockDim.x >> 6);
+  for (uint32_t b = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); b < ncand; b += waves) {
+    if (lane >= 27) continue;
+    const uint32_t g = a.cand_list[b];
+    const int sid = a.nscenes == 1 ? 0 : a.block_scene[g];
+    const SceneDev sc = a.scenes[sid];
+    const int local = (int)g - sc.block_base;
+    const int nb = sc.nb;
+    const int bi = local / (nb * nb), bj = (local / nb) % nb, bk = local % nb;
+    out[(size_t)b * 32 + lane] =
+        neighbour_chunks(a, sc, bi - 1 + lane / 9, bj - 1 + (lane / 3) % 3, bk - 1 + lane % 3);
+  }
+}
+
 __global__ void cand_build_kernel(EngineArgs a, uint32_t* __restrict__ list, uint32_t* __restrict__ count,
                                   uint32_t* __restrict__ flag) {
   const uint32_t gen = *a.cand_gen;
@@ -1347,6 +1371,7 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   if ((rc = alloc((void**)&d_hard, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_cand_list, sizeof(uint32_t) * blocks))) return rc;
   if ((rc = alloc((void**)&d_cand_flag, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_cand_src, sizeof(uint2) * 32 * blocks))) return rc;
   HIPCHK(hipMemset(d_cand_flag, 0, sizeof(uint32_t) * blocks));
   ws_bytes = radix_workspace_bytes(n);
   if ((rc = alloc(&d_ws, ws_bytes))) return rc;
@@ -1375,6 +1400,7 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.flags = d_flags;
   a.cand_list = d_cand_list;
   a.cand_count = d_counters + 4;
+  a.cand_src = d_cand_src;
   a.cand_flag = d_cand_flag;
   a.cand_gen = d_counters + 5;
   a.list = d_list + (size_t)sel * (size_t)total_blocks;
@@ -1465,22 +1491,31 @@ void Engine::enqueue_bin(hipStream_t s) {
   d_sorted_keys = ks;
   hipLaunchKernelGGL(cand_build_kernel, dim3(num_cus * 2), dim3(256), 0, s, args(nullptr, 0, 0), d_cand_list,
                      d_counters + 4, d_cand_flag);
+  hipLaunchKernelGGL(cand_src_kernel, dim3(num_cus * 2), dim3(256), 0, s, args(nullptr, 0, 0), d_cand_src);
 }
 
 void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
   const int g = num_cus * grid_mult;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // timing mode: the events take the kernel's own start and end timestamps
+  // (hipExtLaunchKernel), not those of markers around it
   if (timing) {
     e0 = tevent();
     e1 = tevent();
-    hipEventRecord(e0, s);
   }
+#define MPM_LAUNCH_K(K)                                                                              \
+  do {                                                                                               \
+    if (timing)                                                                                      \
+      hipExtLaunchKernelGGL(K, dim3(g), dim3(ENG_THREADS), 0, s, e0, e1, 0, a);                      \
+    else                                                                                             \
+      hipLaunchKernelGGL(K, dim3(g), dim3(ENG_THREADS), 0, s, a);                                    \
+  } while (0)
 #define MPM_LAUNCH_T(M, PV)                                                                          \
   do {                                                                                               \
     if (M != 2 && spread == 9)                                                                       \
-      hipLaunchKernelGGL((transfer_kernel<M, PV, (M == 2 ? 3 : 9)>), dim3(g), dim3(ENG_THREADS), 0, s, a); \
+      MPM_LAUNCH_K((transfer_kernel<M, PV, (M == 2 ? 3 : 9)>));                                      \
     else                                                                                             \
-      hipLaunchKernelGGL((transfer_kernel<M, PV, 3>), dim3(g), dim3(ENG_THREADS), 0, s, a);        \
+      MPM_LAUNCH_K((transfer_kernel<M, PV, 3>));                                                     \
   } while (0)
   if (chunk == 256) {
     if (mode == 0) MPM_LAUNCH_T(0, 1);
@@ -1496,10 +1531,8 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
     else MPM_LAUNCH_T(2, 4);
   }
 #undef MPM_LAUNCH_T
-  if (timing) {
-    hipEventRecord(e1, s);
-    tpairs.push_back({mode, e0, e1});
-  }
+#undef MPM_LAUNCH_K
+  if (timing) tpairs.push_back({mode, e0, e1});
   // one workgroup per CU: the usual empty list costs each workgroup one load,
   // and a violent step that defers tens of thousands of particles to FP64
   // (seen with dt several times the stable step) is spread over the chip
@@ -1513,12 +1546,10 @@ void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int 
   if (timing) {
     e0 = tevent();
     e1 = tevent();
-    hipEventRecord(e0, s);
-  }
-  hipLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, a, with_sums, len, k);
-  if (timing) {
-    hipEventRecord(e1, s);
+    hipExtLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, e0, e1, 0, a, with_sums, len, k);
     tpairs.push_back({3, e0, e1});
+  } else {
+    hipLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, a, with_sums, len, k);
   }
 }
 

@@ -75,6 +75,7 @@ struct EngineArgs {
   // (global-atomic scatter of drifted particles, the deferred FP64 pass).
   const uint32_t* cand_list;
   const uint32_t* cand_count;
+  const uint2* cand_src;  // per candidate entry: (first chunk, chunk count) of its 27 neighbours, 32 slots
   const uint32_t* cand_flag;  // == *cand_gen: the block is a candidate of the current binning
   const uint32_t* cand_gen;   // binning generation (counters[5])
   uint32_t* list;        // blocks marked during this substep (not candidates)
@@ -138,6 +139,23 @@ __device__ __forceinline__ bool locate3(const float* xp, const SceneDev& sc, int
     fx[d] = s - (float)base[d];
   }
   return ok;
+}
+
+// First staged chunk and chunk count of block (S0, S1, S2) of scene sc
+// (0, 0 outside the scene's blocks), clamped to the staging capacity.
+__device__ __forceinline__ uint2 neighbour_chunks(const EngineArgs& a, const SceneDev& sc, int S0, int S1,
+                                                  int S2) {
+  const int nb = sc.nb;
+  uint32_t c0 = 0, cn = 0;
+  if (S0 >= 0 && S0 < nb && S1 >= 0 && S1 < nb && S2 >= 0 && S2 < nb) {
+    const uint32_t S = (uint32_t)(sc.block_base + (S0 * nb + S1) * nb + S2);
+    // both loads in flight together (bfirst is stale where bcount is 0)
+    cn = a.bcount[S];
+    c0 = a.bfirst[S];
+    c0 = cn ? c0 : 0u;
+    cn = c0 + cn <= a.stage_cap ? cn : (c0 < a.stage_cap ? a.stage_cap - c0 : 0u);
+  }
+  return make_uint2(c0, cn);
 }
 
 // Block g must be in this substep's grid update: candidates always are.
