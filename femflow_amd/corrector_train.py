@@ -272,14 +272,15 @@ class CorrectedRunner:
     is the ``judge_every``-th since the last judged one, or while the runner
     is on probation after a fallback: the same window is then also run with
     plain substeps from the window's start state, and the corrected result is
-    kept only if its relative position error against the plain one,
-    |x - x_plain| / |x_plain|, is at most ``tol``; otherwise the plain result
-    replaces it (a fallback) and the next window is judged too.  Every
-    window, judged or not, must also be healthy -- no stencil left the grid,
-    positions finite and inside the unit domain, speeds below the big step's
-    one-cell-per-step limit -- or it is rerun with plain substeps.  If the
-    plain substeps themselves fail that check the rollout has diverged
-    beyond recovery and ``run`` raises ``RolloutDiverged``.  ``judge_every = 1`` judges every window (the
+    kept only if its relative errors against the plain one (|x - x_plain| /
+    |x_plain|, and the same for v and F) are at most ``tol``; otherwise the
+    plain result replaces it (a fallback) and the next window is judged too.
+    Every window, judged or not, must also be healthy -- no stencil left the
+    grid, positions finite and inside the unit domain, speeds below the big
+    step's one-cell-per-step limit -- or it is rerun with plain substeps.  If
+    the plain substeps themselves fail that check the rollout has diverged
+    beyond recovery and ``run`` raises ``RolloutDiverged``.
+    ``judge_every = 1`` judges every window (the
     error bound holds window by window, at the cost of the plain substeps);
     larger values trade the bound for throughput.  ``stats`` counts the big
     steps kept, the plain substeps run and the fallbacks."""
@@ -320,17 +321,23 @@ class CorrectedRunner:
     def fallbacks(self):
         return self.stats["fallbacks"] + self.stats["unhealthy"]
 
-    def _healthy(self, eng, cur, dt: float) -> bool:
+    def _healthy(self, eng, cur, dt: float, tag: str = "big") -> bool:
         dx = 1.0 / self.cfg.res
         vmax = 0.9 * dx / dt
-        if eng.oob_count():  # reading resets the engine's counter
-            return False
         x = cur["x"]
-        if not bool(torch.isfinite(x).all()):
-            return False
-        if float(x.min()) < 0.0 or float(x.max()) > 1.0:
-            return False
-        return float(cur["v"].norm(dim=1).max()) < vmax
+        if eng.oob_count():  # reading resets the engine's counter
+            why = "left_grid"
+        elif not bool(torch.isfinite(x).all()):
+            why = "non_finite"
+        elif float(x.min()) < 0.0 or float(x.max()) > 1.0:
+            why = "outside_domain"
+        elif not float(cur["v"].norm(dim=1).max()) < vmax:
+            why = "too_fast"
+        else:
+            return True
+        key = f"unhealthy_{tag}_{why}"
+        self.stats[key] = self.stats.get(key, 0) + 1
+        return False
 
     def run(self, s: Dict[str, torch.Tensor], n_big: int, check: bool = True) -> Dict[str, torch.Tensor]:
         """n_big big steps of simulated time from state s (caller order);
@@ -364,13 +371,13 @@ class CorrectedRunner:
                 plain.run(k * cfg.factor, check_oob=False)
                 st["plain_substeps"] += k * cfg.factor
                 ref = _download(plain, start)
-                if not self._healthy(plain, ref, cfg.dt):
+                if not self._healthy(plain, ref, cfg.dt, "plain"):
                     raise RolloutDiverged(f"plain substeps from the state after big step {done} left the "
                                           f"domain or went non-finite", done)
             if judge:
                 st["judged"] += 1
                 since_judged = 0
-                err = rel_position_error(cand["x"], ref["x"])
+                err = judged_error(cand, ref)
                 st["worst_judged_error"] = max(st["worst_judged_error"], err if math.isfinite(err) else 1.0)
                 ok = math.isfinite(err) and err <= self.tol
                 if not ok:
@@ -384,6 +391,15 @@ class CorrectedRunner:
                 probation = True
             done += k
         return cur
+
+
+def judged_error(cand: Dict[str, torch.Tensor], ref: Dict[str, torch.Tensor]) -> float:
+    """The judge's error of a corrected window against the plain substeps
+    from the same start: the largest relative error of positions,
+    velocities and deformation gradients.  Positions alone let a window
+    through whose F has drifted, and the plain substeps that continue from
+    it later blow up."""
+    return max(rel_position_error(cand[k], ref[k]) for k in ("x", "v", "F"))
 
 
 def rel_position_error(x, ref) -> float:

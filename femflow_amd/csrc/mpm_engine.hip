@@ -447,6 +447,9 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
     wg_oob = 0;
     wg_fb = 0;
   }
+  // before any wave counts into them (the first P2G has no barrier ahead of
+  // its particle phase)
+  __syncthreads();
   const uint32_t nch = *a.nchunks;
   const int xcd = blockIdx.x & 7;
   const int l = blockIdx.x >> 3;
