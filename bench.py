@@ -516,11 +516,13 @@ def main():
     # HBM bytes per launch of the fused kernel from the committed rocprofv3
     # PMC summary of this workload (FETCH_SIZE, WRITE_SIZE, calibrated by a
     # plain copy); counters cannot be read inside this run
-    traffic = None
+    traffic = grid_traffic = None
     prof = os.path.join(ROOT, "profiles", "g2p2g_c2_1M_128_traffic.json")
     if os.path.exists(prof):
         with open(prof) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("bytes_per_launch")
+        grid_traffic = tj.get("grid_update_bytes_per_launch")
 
     eng.close()
     batched = batched_bench(args.batched_steps, dev) if args.batched_steps > 0 else None
@@ -563,6 +565,8 @@ def main():
             "kernel": dom, "kernel_ms": avg_s * 1e3,
             "alg_bytes_per_launch": alg_bytes, "active_nodes": active_nodes,
             "grid_blocks_processed": grid_blocks,
+            "grid_update_traffic": grid_traffic,
+            "grid_update_alg_bytes": BYTES_PER_NODE_TRANSFER * 64 * grid_blocks,
         },
         "rebins_in_window": -(-args.steps // args.sort_every),
         "steady_state": steady,

@@ -86,7 +86,7 @@ struct Engine {
   unsigned long long *d_oob = nullptr, *d_fallback = nullptr;
   int4* d_chunks = nullptr;
   uint32_t *d_nchunks = nullptr, *d_keys = nullptr, *d_vals = nullptr,
-           *d_cidx = nullptr, *d_part = nullptr;
+           *d_bocc = nullptr, *d_part = nullptr;
   uint32_t* d_sorted_keys = nullptr;
   // sort permutation of the last rebin, consumed by the next P2G (which
   // moves the particles into binned order); null once consumed

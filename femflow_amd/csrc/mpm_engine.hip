@@ -714,7 +714,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         const float4 val = make_float4((float)((double)q0 * inv[0]), (float)((double)q1 * inv[1]),
                                        (float)((double)q2 * inv[2]), (float)((double)q3 * inv[3]));
         const bool to_stage = staged && e0 < STAGE && e1 < STAGE && e2 < STAGE;
-        if (to_stage) stage[(e0 * STAGE + e1) * STAGE + e2] = val;
+        if (to_stage) stage[stage_slot(e0, e1, e2)] = val;
         // every arena node lies in one of the 27 blocks around this chunk's
         // block, all candidates of the grid update: no marking needed
         if (!to_stage && (q0 != 0 || q1 != 0 || q2 != 0 || q3 != 0)) {
@@ -916,8 +916,9 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       // arena index of this node in the source's arena; index 7 (the
       // unstaged eighth layer) came through the global accumulator
       const int x0 = o0 + 1 - 4 * s0, x1 = o1 + 1 - 4 * s1, x2 = o2 + 1 - 4 * s2;
-      ai[sn] = (x0 * STAGE + x1) * STAGE + x2;
-      if (x0 == STAGE || x1 == STAGE || x2 == STAGE) cn[sn] = 0u;
+      const bool unstaged = x0 == STAGE || x1 == STAGE || x2 == STAGE;
+      ai[sn] = unstaged ? 0 : stage_slot(x0, x1, x2);
+      if (unstaged) cn[sn] = 0u;
     }
     // up to three chunks per source are loaded in one round trip (a block
     // of ~20 particles per cell has three 512-particle chunks) and summed
@@ -1049,8 +1050,11 @@ __device__ __forceinline__ int cell_of(float x, float inv_dx, int res) {
   return b < 0 ? 0 : (b > res ? res : b);
 }
 
-__global__ void engine_keys_kernel(EngineArgs a, int64_t n, uint32_t* __restrict__ keys) {
+__global__ void engine_keys_kernel(EngineArgs a, int64_t n, uint32_t* __restrict__ keys,
+                                   uint32_t* __restrict__ bocc, int64_t nblocks) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // block occupancy of the new binning starts cleared (set by block_bounds)
+  for (int64_t j = i; j < nblocks; j += (int64_t)gridDim.x * blockDim.x) bocc[j] = 0u;
   if (i >= n) return;
   const SceneDev sc = a.scenes[a.s.attr[i] >> 8];
   const int b0 = cell_of(a.s.x[tix(i, 3, 0)], sc.inv_dx, sc.res);
@@ -1061,31 +1065,16 @@ __global__ void engine_keys_kernel(EngineArgs a, int64_t n, uint32_t* __restrict
 }
 
 __global__ void block_bounds_kernel(const uint32_t* __restrict__ keys, int64_t n,
-                                    uint32_t* __restrict__ bstart, uint32_t* __restrict__ bend) {
+                                    uint32_t* __restrict__ bstart, uint32_t* __restrict__ bend,
+                                    uint32_t* __restrict__ bocc) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t blk = keys[i] >> 6;
-  if (i == 0 || (keys[i - 1] >> 6) != blk) bstart[blk] = (uint32_t)i;
-  if (i == n - 1 || (keys[i + 1] >> 6) != blk) bend[blk] = (uint32_t)(i + 1);
-}
-
-__global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
-                                   const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ idx,
-                                   const uint32_t* __restrict__ bend, const int* __restrict__ block_scene,
-                                   int chunk, int4* __restrict__ chunks, uint32_t* __restrict__ bfirst,
-                                   uint32_t* __restrict__ bcount) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if ((((uint32_t)i - bstart[keys[i] >> 6]) % (uint32_t)chunk) == 0u) {
-    const uint32_t blk = keys[i] >> 6;
-    const uint32_t cnt = bend[blk] - (uint32_t)i;
-    chunks[idx[i]] = make_int4((int)i, (int)(cnt < (uint32_t)chunk ? cnt : (uint32_t)chunk), (int)blk,
-                               block_scene[blk]);
-    if (i == 0 || (keys[i - 1] >> 6) != blk) {  // first chunk of the block
-      bfirst[blk] = idx[i];
-      bcount[blk] = (cnt + (uint32_t)chunk - 1u) / (uint32_t)chunk;
-    }
+  if (i == 0 || (keys[i - 1] >> 6) != blk) {
+    bstart[blk] = (uint32_t)i;
+    bocc[blk] = 1u;
   }
+  if (i == n - 1 || (keys[i + 1] >> 6) != blk) bend[blk] = (uint32_t)(i + 1);
 }
 
 // Candidate blocks of the grid update for the new binning (the segment's
@@ -1094,6 +1083,24 @@ __global__ void chunk_write_kernel(const uint32_t* __restrict__ keys, int64_t n,
 // tag of a block appends it, so each block is listed once.  A block is a
 // candidate while its tag equals the current generation, so the previous
 // list needs no clearing.
+__global__ void cand_build_kernel(EngineArgs a, uint32_t* __restrict__ list, uint32_t* __restrict__ count,
+                                  uint32_t* __restrict__ flag) {
+  const uint32_t gen = *a.cand_gen;
+  const uint32_t n = *a.nchunks * 27u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int4 ch = a.chunks[i / 27u];
+    const int nbh = (int)(i % 27u);
+    const SceneDev sc = a.scenes[ch.w];
+    const int local = ch.z - sc.block_base;
+    const int nb = sc.nb;
+    const int bi = local / (nb * nb) - 1 + nbh / 9, bj = (local / nb) % nb - 1 + (nbh / 3) % 3,
+              bk = local % nb - 1 + nbh % 3;
+    if (bi < 0 || bj < 0 || bk < 0 || bi >= nb || bj >= nb || bk >= nb) continue;
+    const uint32_t g = (uint32_t)(sc.block_base + (bi * nb + bj) * nb + bk);
+    if (flag[g] != gen && atomicExch(flag + g, gen) != gen) list[atomicAdd(count, 1u)] = g;
+  }
+}
+
 // Per candidate entry (one wave each, lanes 0..26): the staged chunks of
 // its 27 neighbouring blocks.  They are fixed until the next rebin, so the
 // grid update reads them together with the entry instead of looking them
@@ -1112,24 +1119,6 @@ __global__ void cand_src_kernel(EngineArgs a, uint2* __restrict__ out) {
     const int bi = local / (nb * nb), bj = (local / nb) % nb, bk = local % nb;
     out[(size_t)b * 32 + lane] =
         neighbour_chunks(a, sc, bi - 1 + lane / 9, bj - 1 + (lane / 3) % 3, bk - 1 + lane % 3);
-  }
-}
-
-__global__ void cand_build_kernel(EngineArgs a, uint32_t* __restrict__ list, uint32_t* __restrict__ count,
-                                  uint32_t* __restrict__ flag) {
-  const uint32_t gen = *a.cand_gen;
-  const uint32_t n = *a.nchunks * 27u;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    const int4 ch = a.chunks[i / 27u];
-    const int nbh = (int)(i % 27u);
-    const SceneDev sc = a.scenes[ch.w];
-    const int local = ch.z - sc.block_base;
-    const int nb = sc.nb;
-    const int bi = local / (nb * nb) - 1 + nbh / 9, bj = (local / nb) % nb - 1 + (nbh / 3) % 3,
-              bk = local % nb - 1 + nbh % 3;
-    if (bi < 0 || bj < 0 || bk < 0 || bi >= nb || bj >= nb || bk >= nb) continue;
-    const uint32_t g = (uint32_t)(sc.block_base + (bi * nb + bj) * nb + bk);
-    if (flag[g] != gen && atomicExch(flag + g, gen) != gen) list[atomicAdd(count, 1u)] = g;
   }
 }
 
@@ -1369,8 +1358,8 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   HIPCHK(hipMemset(d_bcount, 0, sizeof(uint32_t) * blocks));
   if ((rc = alloc((void**)&d_keys, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_vals, sizeof(uint32_t) * n))) return rc;
-  if ((rc = alloc((void**)&d_cidx, sizeof(uint32_t) * n))) return rc;
-  if ((rc = alloc((void**)&d_part, sizeof(uint32_t) * (n / 4096 + 4)))) return rc;
+  if ((rc = alloc((void**)&d_bocc, sizeof(uint32_t) * blocks))) return rc;
+  if ((rc = alloc((void**)&d_part, sizeof(uint32_t) * chunk_list_part_words(blocks)))) return rc;
   if ((rc = alloc((void**)&d_hard, sizeof(uint32_t) * n))) return rc;
   if ((rc = alloc((void**)&d_cand_list, sizeof(uint32_t) * blocks))) return rc;
   if ((rc = alloc((void**)&d_cand_flag, sizeof(uint32_t) * blocks))) return rc;
@@ -1477,7 +1466,8 @@ void Engine::enqueue_bin(hipStream_t s) {
     return;
   }
   EngineArgs a = args(nullptr, 0, 0);
-  hipLaunchKernelGGL(engine_keys_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, a, n, d_keys);
+  hipLaunchKernelGGL(engine_keys_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, a, n, d_keys, d_bocc,
+                     (int64_t)total_blocks);
   uint32_t *ks, *vs;
   const uint32_t max_key = (uint32_t)(total_blocks * 64 - 1);
   radix_sort(d_keys, d_vals, n, max_key, d_ws, true, &ks, &vs, s);
@@ -1485,12 +1475,9 @@ void Engine::enqueue_bin(hipStream_t s) {
   // reads slot perm[p] of the current buffer and writes slot p of the other
   cur ^= 1;
   pending_perm = vs;
-  hipLaunchKernelGGL(block_bounds_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, d_bend);
-  chunk_index(ks, d_bstart, n, chunk, d_cidx, d_part, d_nchunks, s);
-  hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(total_blocks, 256)), dim3(256), 0, s, d_bcount,
-                     (int64_t)total_blocks, 0u);
-  hipLaunchKernelGGL(chunk_write_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, d_cidx,
-                     d_bend, d_block_scene, chunk, d_chunks, d_bfirst, d_bcount);
+  hipLaunchKernelGGL(block_bounds_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, ks, n, d_bstart, d_bend, d_bocc);
+  chunk_list(d_bocc, d_bstart, d_bend, d_block_scene, (int64_t)total_blocks, chunk, d_part, d_chunks, d_bfirst,
+             d_bcount, d_nchunks, s);
   d_sorted_keys = ks;
   hipLaunchKernelGGL(cand_build_kernel, dim3(num_cus * 2), dim3(256), 0, s, args(nullptr, 0, 0), d_cand_list,
                      d_counters + 4, d_cand_flag);

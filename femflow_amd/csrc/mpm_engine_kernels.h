@@ -25,6 +25,26 @@ constexpr int ARENA_NODES = ARENA * ARENA * ARENA;
 // global atomics.
 constexpr int STAGE = 7;
 constexpr int STAGE_NODES = STAGE * STAGE * STAGE;
+
+// Slot of staged node (x0, x1, x2) (each 0..6) in its chunk's staging
+// record.  Per axis the seven nodes split by the block they belong to:
+// node 0 lies in the block before the chunk's own (part 0), nodes 1..4 in
+// its own (part 1), nodes 5..6 in the block after (part 2).  The record
+// stores the 27 part boxes one after another (part-major, sizes 1, 4, 2 per
+// axis), each in row-major order, so the grid update of one block reads
+// each source chunk's contribution to it as one contiguous run of slots in
+// its lanes' order.
+__device__ __forceinline__ int stage_slot(int x0, int x1, int x2) {
+  const int p0 = x0 == 0 ? 0 : (x0 < 5 ? 1 : 2), p1 = x1 == 0 ? 0 : (x1 < 5 ? 1 : 2),
+            p2 = x2 == 0 ? 0 : (x2 < 5 ? 1 : 2);
+  // part start (0, 1, 5) and size (1, 4, 2)
+  const int s0 = p0 == 0 ? 0 : (p0 == 1 ? 1 : 5), s1 = p1 == 0 ? 0 : (p1 == 1 ? 1 : 5),
+            s2 = p2 == 0 ? 0 : (p2 == 1 ? 1 : 5);
+  const int z0 = p0 == 0 ? 1 : (p0 == 1 ? 4 : 2), z1 = p1 == 0 ? 1 : (p1 == 1 ? 4 : 2),
+            z2 = p2 == 0 ? 1 : (p2 == 1 ? 4 : 2);
+  const int base = s0 * (STAGE * STAGE) + z0 * s1 * STAGE + z0 * z1 * s2;
+  return base + ((x0 - s0) * z1 + (x1 - s1)) * z2 + (x2 - s2);
+}
 constexpr int MAX_SCENES = 4096;
 
 struct SceneDev {

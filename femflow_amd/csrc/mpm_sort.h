@@ -10,11 +10,15 @@ namespace mpm {
 
 size_t radix_workspace_bytes(int64_t n);
 int radix_passes(uint32_t max_key);
-// Chunk numbering of the binned particles: idx[i] = number of chunk starts
-// before sorted key i (a start: offset from its block's first key is a
-// multiple of chunk); *nchunks = their total.
-void chunk_index(const uint32_t* keys, const uint32_t* bstart, int64_t n, int chunk, uint32_t* idx,
-                 uint32_t* part, uint32_t* nchunks, hipStream_t st);
+// Chunk list of a binning: occupied block b (bocc[b] != 0, particles
+// bstart[b] .. bend[b]) gets ceil(count / chunk) chunks (start, count, b,
+// block_scene[b]), numbered after those of the blocks before it; bfirst[b]
+// and bcount[b] are its first chunk and chunk count (0 for empty blocks);
+// *nchunks = the total.  part: chunk_list_part_words(nblocks) words.
+size_t chunk_list_part_words(int64_t nblocks);
+void chunk_list(const uint32_t* bocc, const uint32_t* bstart, const uint32_t* bend, const int* block_scene,
+                int64_t nblocks, int chunk, uint32_t* part, int4* chunks, uint32_t* bfirst, uint32_t* bcount,
+                uint32_t* nchunks, hipStream_t st);
 void radix_sort(uint32_t* keys, uint32_t* vals, int64_t n, uint32_t max_key, void* ws,
                 bool iota_values, uint32_t** kout, uint32_t** vout, hipStream_t st);
 void bin_keys(const float* x, int64_t n, int dim, int res, float inv_dx, int layout, uint32_t* keys,

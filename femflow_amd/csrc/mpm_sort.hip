@@ -177,59 +177,60 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
   return incl - v;
 }
 
-// ------------------------------------------------------- chunk starts ----
-// Sorted key i starts a chunk when its offset from its block's first key is
-// a multiple of `chunk`.  The exclusive scan of those flags numbers the
-// chunks; it runs in two launches: per-tile counts, then each tile adds the
-// counts of the tiles before it (at most a few hundred) and scans itself.
-__device__ __forceinline__ uint32_t chunk_start_flag(const uint32_t* __restrict__ keys,
-                                                     const uint32_t* __restrict__ bstart, int64_t i,
-                                                     int chunk) {
-  return ((((uint32_t)i - bstart[keys[i] >> 6]) % (uint32_t)chunk) == 0u) ? 1u : 0u;
+// ------------------------------------------------------- chunk list ----
+// Chunk list of the new binning, numbered over blocks: block b holds
+// ceil(count_b / chunk) chunks, numbered after those of the blocks before
+// it.  Two launches over tiles of SCAN_THREADS blocks, one block per thread
+// (coalesced reads, a few hundred tiles for a 128^3 grid): per-tile chunk
+// counts, then each tile adds the counts of the tiles before it, scans its
+// own blocks and writes their chunks, first chunk and chunk count.
+__device__ __forceinline__ uint32_t block_chunks(const uint32_t* __restrict__ bocc,
+                                                 const uint32_t* __restrict__ bstart,
+                                                 const uint32_t* __restrict__ bend, int64_t b, int chunk) {
+  return bocc[b] ? (bend[b] - bstart[b] + (uint32_t)chunk - 1u) / (uint32_t)chunk : 0u;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void chunk_count_kernel(const uint32_t* __restrict__ keys,
-                                                                   const uint32_t* __restrict__ bstart,
-                                                                   int64_t n, int chunk,
-                                                                   uint32_t* __restrict__ part) {
+__global__ __launch_bounds__(SCAN_THREADS) void block_tile_kernel(const uint32_t* __restrict__ bocc,
+                                                                  const uint32_t* __restrict__ bstart,
+                                                                  const uint32_t* __restrict__ bend,
+                                                                  int64_t nblocks, int chunk,
+                                                                  uint32_t* __restrict__ part) {
   __shared__ uint32_t sh[SCAN_THREADS];
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-  uint32_t s = 0;
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; ++i) s += (base + i < n) ? chunk_start_flag(keys, bstart, base + i, chunk) : 0u;
+  const int64_t b = (int64_t)blockIdx.x * SCAN_THREADS + threadIdx.x;
+  const uint32_t c = b < nblocks ? block_chunks(bocc, bstart, bend, b, chunk) : 0u;
   uint32_t tot;
-  block_exclusive_scan(s, sh, &tot);
+  block_exclusive_scan(c, sh, &tot);
   if (threadIdx.x == 0) part[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void chunk_index_kernel(const uint32_t* __restrict__ keys,
-                                                                   const uint32_t* __restrict__ bstart,
-                                                                   int64_t n, int chunk,
-                                                                   const uint32_t* __restrict__ part,
-                                                                   uint32_t* __restrict__ idx,
-                                                                   uint32_t* __restrict__ nchunks) {
+__global__ __launch_bounds__(SCAN_THREADS) void block_chunks_kernel(
+    const uint32_t* __restrict__ bocc, const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ bend,
+    const int* __restrict__ block_scene, int64_t nblocks, int chunk, const uint32_t* __restrict__ part,
+    int4* __restrict__ chunks, uint32_t* __restrict__ bfirst, uint32_t* __restrict__ bcount,
+    uint32_t* __restrict__ nchunks) {
   __shared__ uint32_t sh[SCAN_THREADS];
-  // chunks started in the tiles before this one
   uint32_t before = 0;
-  for (uint32_t b = threadIdx.x; b < blockIdx.x; b += SCAN_THREADS) before += part[b];
+  for (uint32_t q = threadIdx.x; q < blockIdx.x; q += SCAN_THREADS) before += part[q];
   uint32_t tot_before;
   block_exclusive_scan(before, sh, &tot_before);
-  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
-  uint32_t f[SCAN_ITEMS];
-  uint32_t s = 0;
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    f[i] = (base + i < n) ? chunk_start_flag(keys, bstart, base + i, chunk) : 0u;
-    s += f[i];
-  }
+  const int64_t b = (int64_t)blockIdx.x * SCAN_THREADS + threadIdx.x;
+  const uint32_t c = b < nblocks ? block_chunks(bocc, bstart, bend, b, chunk) : 0u;
   uint32_t tot;
-  uint32_t run = block_exclusive_scan(s, sh, &tot) + tot_before;
-#pragma unroll
-  for (int i = 0; i < SCAN_ITEMS; ++i) {
-    if (base + i < n) idx[base + i] = run;
-    run += f[i];
+  const uint32_t run = block_exclusive_scan(c, sh, &tot) + tot_before;
+  if (b < nblocks) {
+    bfirst[b] = run;
+    bcount[b] = c;
+    if (c) {
+      const uint32_t p0 = bstart[b], cnt = bend[b] - p0;
+      const int sc = block_scene[b];
+      for (uint32_t j = 0; j < c; ++j) {
+        const uint32_t left = cnt - j * (uint32_t)chunk;
+        chunks[run + j] = make_int4((int)(p0 + j * (uint32_t)chunk),
+                                    (int)(left < (uint32_t)chunk ? left : (uint32_t)chunk), (int)b, sc);
+      }
+    }
   }
-  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) *nchunks = run;
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) *nchunks = run + c;
 }
 
 // ---------------------------------------------------------------- keys ----
@@ -279,14 +280,17 @@ size_t radix_workspace_bytes(int64_t n) {
   return sizeof(uint32_t) * (4 * (size_t)n + 2 * (size_t)(hist > n ? hist : n) + parts + 256 + 64);
 }
 
-void chunk_index(const uint32_t* keys, const uint32_t* bstart, int64_t n, int chunk, uint32_t* idx,
-                 uint32_t* part, uint32_t* nchunks, hipStream_t st) {
-  if (n <= 0) return;
-  const int64_t nb = cdiv(n, SCAN_TILE);
-  hipLaunchKernelGGL(chunk_count_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, keys, bstart, n, chunk,
-                     part);
-  hipLaunchKernelGGL(chunk_index_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, st, keys, bstart, n, chunk,
-                     part, idx, nchunks);
+size_t chunk_list_part_words(int64_t nblocks) { return (size_t)cdiv(nblocks, SCAN_THREADS) + 1; }
+
+void chunk_list(const uint32_t* bocc, const uint32_t* bstart, const uint32_t* bend, const int* block_scene,
+                int64_t nblocks, int chunk, uint32_t* part, int4* chunks, uint32_t* bfirst, uint32_t* bcount,
+                uint32_t* nchunks, hipStream_t st) {
+  if (nblocks <= 0) return;
+  const unsigned tiles = (unsigned)cdiv(nblocks, SCAN_THREADS);
+  hipLaunchKernelGGL(block_tile_kernel, dim3(tiles), dim3(SCAN_THREADS), 0, st, bocc, bstart, bend, nblocks,
+                     chunk, part);
+  hipLaunchKernelGGL(block_chunks_kernel, dim3(tiles), dim3(SCAN_THREADS), 0, st, bocc, bstart, bend,
+                     block_scene, nblocks, chunk, part, chunks, bfirst, bcount, nchunks);
 }
 
 int radix_passes(uint32_t max_key) {
