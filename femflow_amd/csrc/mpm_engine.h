@@ -127,8 +127,12 @@ struct Engine {
   void enqueue_frame(hipStream_t s, float* frames, int64_t step);
   void enqueue_state(hipStream_t s, const StateSink* sink, int64_t step);
   int positions(float* x);
-  void launch_transfer(int mode, hipStream_t s, const EngineArgs& a);
-  void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k);
+  // separate_deferred: finish the FP64-deferred particles in their own
+  // launch right after the transfer (needed where something reads the
+  // particles before the grid update, and after the last G2P); otherwise
+  // the following launch_grid(..., deferred = 1) finishes them
+  void launch_transfer(int mode, hipStream_t s, const EngineArgs& a, bool separate_deferred = true);
+  void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k, int deferred = 0);
   void begin_segment(hipStream_t s, int len, int with_sums, int bin);
   void fill_u32(hipStream_t s, uint32_t* p, int64_t count, uint32_t v);
   void launch_cellf(hipStream_t s, const EngineArgs& a);

@@ -741,97 +741,106 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
 // ------------------------------------------------------- deferred (FP64) ----
 // Particles the FP32 transfer declined: snow plasticity whose SVD is too
 // ill-conditioned for FP32 (s_min < 1e-2 s_max), and stresses whose FP32
-// polar iteration did not settle (F near singular).  They are finished here
-// in FP64 (Jacobi SVD, Newton polar with SVD fallback) and scattered with
-// global atomics.  The last workgroup to finish clears the list.
+// polar iteration did not settle (F near singular).  They are finished in
+// FP64 (Jacobi SVD, Newton polar with SVD fallback) and scattered with
+// global atomics, either by the grid update that follows the transfer (its
+// first-started workgroup, before any workgroup reads the grid) or, where
+// something reads the particles in between, by deferred_kernel.
+__device__ __forceinline__ void finish_deferred(const EngineArgs& a, uint32_t e) {
+  const int64_t p = e >> 2;
+  const uint32_t attr = a.s.attr[p];
+  int mid = (int)(attr & 255u);
+  mid = mid < a.nmat ? mid : a.nmat - 1;
+  const MatParams mp = a.mats[mid];
+  const SceneDev sc = a.scenes[attr >> 8];
+  double F[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) F[k] = a.s.F[tix(p, 9, k)];
+  double Jp = a.s.Jp[p];
+  if ((e & 1u) && mp.kind == MPM_SNOW) {
+    double U[9], sv[3], V[9];
+    svd3<double>(F, U, sv, V, 10);
+    const double oldJ = det3(F);
+    const double lo = 1.0 - (double)mp.theta_c, hi = 1.0 + (double)mp.theta_s;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sv[c] = tmin(tmax(sv[c], lo), hi);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        F[3 * r + c] = U[3 * r] * sv[0] * V[3 * c] + U[3 * r + 1] * sv[1] * V[3 * c + 1] +
+                       U[3 * r + 2] * sv[2] * V[3 * c + 2];
+    Jp = tmin(tmax(Jp * oldJ / (det3(F) + 1e-10), 0.6), 20.0);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.s.F[tix(p, 9, k)] = (float)F[k];
+    a.s.Jp[p] = (float)Jp;
+  }
+  if (e & 2u) return;
+  float xf[3];
+  loadk<3>(a.s.x, p, xf);
+  int base[3];
+  float fxf[3];
+  if (!locate3(xf, sc, base, fxf)) {
+    atomicAdd(a.oob, 1ull);
+    return;
+  }
+  double C[9], v[3], fx[3], w[3][3];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) C[k] = a.s.C[tix(p, 9, k)];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    v[d] = a.s.v[tix(p, 3, d)];
+    fx[d] = fxf[d];
+    bspline(fx[d], w[d]);
+  }
+  const double m = a.s.mass[p];
+  double mu, lam, A[9];
+  lame<double>(mp.kind, (double)mp.hardening, (double)a.s.mu0[p], (double)a.s.lam0[p], Jp, mu, lam);
+  affine3<double>(F, C, m, mu, lam, mp.kind, (double)sc.inv_dx, (double)sc.dt, (double)sc.vol,
+                  sc.quirks == MPM_QUIRKS_TEXTBOOK, A);
+  const double dx = sc.dx;
+#pragma unroll
+  for (int i0 = 0; i0 < 3; ++i0)
+#pragma unroll
+    for (int j0 = 0; j0 < 3; ++j0)
+#pragma unroll
+      for (int k0 = 0; k0 < 3; ++k0) {
+        const double wt = w[0][i0] * w[1][j0] * w[2][k0];
+        const double dp[3] = {(i0 - fx[0]) * dx, (j0 - fx[1]) * dx, (k0 - fx[2]) * dx};
+        float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, base[0] + i0, base[1] + j0, base[2] + k0));
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          atomicAdd(dst + d, (float)(wt * (m * v[d] + A[3 * d] * dp[0] + A[3 * d + 1] * dp[1] + A[3 * d + 2] * dp[2])));
+        atomicAdd(dst + 3, (float)(wt * m));
+      }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int i = base[0] + ((c >> 2) & 1) * 2, j = base[1] + ((c >> 1) & 1) * 2, k = base[2] + (c & 1) * 2;
+    mark_active(a, (uint32_t)(sc.block_base + ((i >> 2) * sc.nb + (j >> 2)) * sc.nb + (k >> 2)));
+  }
+}
+
 __global__ __launch_bounds__(ENG_THREADS) void deferred_kernel(EngineArgs a) {
-  const uint32_t count = a.counters[3];
+  const uint32_t count = *a.hard_count;
   uint32_t done_here = 0;
   for (uint32_t i = blockIdx.x * ENG_THREADS + threadIdx.x; i < count; i += gridDim.x * ENG_THREADS) {
-    const uint32_t e = a.hard_list[i];
-    const int64_t p = e >> 2;
+    finish_deferred(a, a.hard_list[i]);
     ++done_here;
-    const uint32_t attr = a.s.attr[p];
-    int mid = (int)(attr & 255u);
-    mid = mid < a.nmat ? mid : a.nmat - 1;
-    const MatParams mp = a.mats[mid];
-    const SceneDev sc = a.scenes[attr >> 8];
-    double F[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) F[k] = a.s.F[tix(p, 9, k)];
-    double Jp = a.s.Jp[p];
-    if ((e & 1u) && mp.kind == MPM_SNOW) {
-      double U[9], sv[3], V[9];
-      svd3<double>(F, U, sv, V, 10);
-      const double oldJ = det3(F);
-      const double lo = 1.0 - (double)mp.theta_c, hi = 1.0 + (double)mp.theta_s;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) sv[c] = tmin(tmax(sv[c], lo), hi);
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          F[3 * r + c] = U[3 * r] * sv[0] * V[3 * c] + U[3 * r + 1] * sv[1] * V[3 * c + 1] +
-                         U[3 * r + 2] * sv[2] * V[3 * c + 2];
-      Jp = tmin(tmax(Jp * oldJ / (det3(F) + 1e-10), 0.6), 20.0);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) a.s.F[tix(p, 9, k)] = (float)F[k];
-      a.s.Jp[p] = (float)Jp;
-    }
-    if (e & 2u) continue;
-    float xf[3];
-    loadk<3>(a.s.x, p, xf);
-    int base[3];
-    float fxf[3];
-    if (!locate3(xf, sc, base, fxf)) {
-      atomicAdd(a.oob, 1ull);
-      continue;
-    }
-    double C[9], v[3], fx[3], w[3][3];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) C[k] = a.s.C[tix(p, 9, k)];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      v[d] = a.s.v[tix(p, 3, d)];
-      fx[d] = fxf[d];
-      bspline(fx[d], w[d]);
-    }
-    const double m = a.s.mass[p];
-    double mu, lam, A[9];
-    lame<double>(mp.kind, (double)mp.hardening, (double)a.s.mu0[p], (double)a.s.lam0[p], Jp, mu, lam);
-    affine3<double>(F, C, m, mu, lam, mp.kind, (double)sc.inv_dx, (double)sc.dt, (double)sc.vol,
-                    sc.quirks == MPM_QUIRKS_TEXTBOOK, A);
-    const double dx = sc.dx;
-#pragma unroll
-    for (int i0 = 0; i0 < 3; ++i0)
-#pragma unroll
-      for (int j0 = 0; j0 < 3; ++j0)
-#pragma unroll
-        for (int k0 = 0; k0 < 3; ++k0) {
-          const double wt = w[0][i0] * w[1][j0] * w[2][k0];
-          const double dp[3] = {(i0 - fx[0]) * dx, (j0 - fx[1]) * dx, (k0 - fx[2]) * dx};
-          float* dst = reinterpret_cast<float*>(a.acc + node_addr(sc, base[0] + i0, base[1] + j0, base[2] + k0));
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-            atomicAdd(dst + d, (float)(wt * (m * v[d] + A[3 * d] * dp[0] + A[3 * d + 1] * dp[1] + A[3 * d + 2] * dp[2])));
-          atomicAdd(dst + 3, (float)(wt * m));
-        }
-#pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const int i = base[0] + ((c >> 2) & 1) * 2, j = base[1] + ((c >> 1) & 1) * 2, k = base[2] + (c & 1) * 2;
-      mark_active(a, (uint32_t)(sc.block_base + ((i >> 2) * sc.nb + (j >> 2)) * sc.nb + (k >> 2)));
-    }
   }
   if (done_here) atomicAdd(a.fallback, (unsigned long long)done_here);
 }
 
-// Start of a segment: clear both marked-block list counters and the deferred
-// list, advance the per-substep sums index by the segment length, and, when
+// Start of a segment: clear both marked-block list counters, both deferred
+// counts and the grid update's start ticket, advance the per-substep sums
+// index by the segment length, and, when
 // the segment rebins, start a new candidate list (count 0, next generation).
 __global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums, int bin) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     counters[0] = 0u;
     counters[1] = 0u;
+    counters[3] = 0u;
+    counters[6] = 0u;
+    counters[7] = 0u;
     counters[3] = 0u;
     if (with_sums) counters[2] += (uint32_t)len;
     if (bin) {
@@ -851,12 +860,54 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Deferred particles left by the transfer before this grid update: the
+// workgroup that starts first (a ticket, so it is resident) finishes them
+// and then sets bit 31 of the count; the others wait for that bit before
+// they read the grid.  A count of 0 (the usual case) costs one load.  The
+// wait is bounded; running out sets the error counter instead of hanging.
+constexpr uint32_t DEFER_DONE = 0x80000000u;
+constexpr uint32_t DEFER_SPIN_CAP = 1u << 24;
+
+__device__ __forceinline__ void grid_deferred(const EngineArgs& a) {
+  __shared__ uint32_t ticket_s;
+  // written by the transfer before this launch; only its bit 31 changes
+  // during it, so every thread sees 0 or every thread sees nonzero
+  const uint32_t count = *a.hard_count;
+  if (count == 0u) return;
+  if (threadIdx.x == 0) ticket_s = atomicAdd(a.defer_ticket, 1u);
+  __syncthreads();
+  const uint32_t ticket = ticket_s;
+  if (ticket == 0u && !(count & DEFER_DONE)) {
+    const uint32_t c = count & ~DEFER_DONE;
+    for (uint32_t i = threadIdx.x; i < c; i += ENG_THREADS) finish_deferred(a, a.hard_list[i]);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      atomicAdd(a.fallback, (unsigned long long)c);
+      __threadfence();
+      __hip_atomic_store(a.hard_count, c | DEFER_DONE, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  } else if (threadIdx.x == 0) {
+    uint32_t spins = 0;
+    while (!(__hip_atomic_load(a.hard_count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & DEFER_DONE)) {
+      __builtin_amdgcn_s_sleep(2);
+      if (++spins >= DEFER_SPIN_CAP) {
+        atomicAdd(a.oob, 1ull << 40);  // reported as an error by the host
+        break;
+      }
+    }
+    __threadfence();
+  }
+  // the last workgroup to take a ticket resets it for the next launch
+  if (threadIdx.x == 0 && ticket == gridDim.x - 1u) __hip_atomic_store(a.defer_ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, int with_sums, int len,
-                                                                  int k) {
+                                                                  int k, int deferred) {
   __shared__ double red[ENG_THREADS / 64][8];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *a.other_count = 0u;  // the list the next substep appends to
-    a.counters[3] = 0u;   // deferred list (its pass has run)
+    *a.other_hard = 0u;   // the deferred count the next substep appends to
   }
   const int t = threadIdx.x;
   const int wv = t >> 6, inner = t & 63;
@@ -865,13 +916,17 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
   // dependent loads (list entry, neighbour chunk slots, staged arenas), so
   // the first candidate entry is read together with the counts (every slot
   // below list_cap is addressable; entries past the count are ignored), and
-  // a single-scene engine skips the block -> scene lookup.
+  // a single-scene engine skips the block -> scene lookup.  The candidate
+  // list and its table are fixed since the rebin, so they are read ahead of
+  // the deferred pass; the marked list, the accumulator and the staged
+  // arenas only after it.
   const bool one_scene = a.nscenes == 1;
   const uint32_t stride = gridDim.x * 4u;
   uint32_t b = blockIdx.x * 4u + (uint32_t)wv;
   uint32_t g_next = b < a.list_cap ? a.cand_list[b] : 0u;
   uint2 src_next = b < a.list_cap && inner < 27 ? a.cand_src[(size_t)b * 32 + inner] : make_uint2(0u, 0u);
   const uint32_t ncand = *a.cand_count;
+  if (deferred) grid_deferred(a);
   const uint32_t count = ncand + *a.list_count;
   for (; b < count; b += stride) {
     const bool cand = b < ncand;
@@ -1407,7 +1462,9 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.list_cap = (uint32_t)total_blocks;
   a.oob = d_oob;
   a.hard_list = d_hard;
-  a.hard_count = d_counters + 3;
+  a.hard_count = d_counters + (sel ? 6 : 3);
+  a.other_hard = d_counters + (sel ? 3 : 6);
+  a.defer_ticket = d_counters + 7;
   a.fallback = d_fallback;
   a.sums = sums;
   a.max_steps = max_steps;
@@ -1484,7 +1541,7 @@ void Engine::enqueue_bin(hipStream_t s) {
   hipLaunchKernelGGL(cand_src_kernel, dim3(num_cus * 2), dim3(256), 0, s, args(nullptr, 0, 0), d_cand_src);
 }
 
-void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
+void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a, bool separate_deferred) {
   const int g = num_cus * grid_mult;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // timing mode: the events take the kernel's own start and end timestamps
@@ -1526,20 +1583,22 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a) {
   // one workgroup per CU: the usual empty list costs each workgroup one load,
   // and a violent step that defers tens of thousands of particles to FP64
   // (seen with dt several times the stable step) is spread over the chip
-  hipLaunchKernelGGL(deferred_kernel, dim3(num_cus), dim3(ENG_THREADS), 0, s, a);
+  if (separate_deferred || mode == 2)
+    hipLaunchKernelGGL(deferred_kernel, dim3(num_cus), dim3(ENG_THREADS), 0, s, a);
 }
 
-void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k) {
+void Engine::launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k, int deferred) {
   // with sums every workgroup ends with 7 FP64 atomics on the same 7 words
   const int g = with_sums ? num_cus : num_cus * 8;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (timing) {
     e0 = tevent();
     e1 = tevent();
-    hipExtLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, e0, e1, 0, a, with_sums, len, k);
+    hipExtLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, e0, e1, 0, a, with_sums, len, k,
+                          deferred);
     tpairs.push_back({3, e0, e1});
   } else {
-    hipLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, a, with_sums, len, k);
+    hipLaunchKernelGGL(grid_update_kernel, dim3(g), dim3(ENG_THREADS), 0, s, a, with_sums, len, k, deferred);
   }
 }
 
@@ -1586,21 +1645,24 @@ void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int
     if (sink && sink->grids) a.grid_rec = sink->grids + (size_t)(step0 + k) * 7 * rec_nn;
     return a;
   };
+  // the deferred FP64 particles are finished by the grid update unless the
+  // particles are read in between (cell-F sums, per-substep state output)
+  const bool sep = corr_on || sink != nullptr;
   {
     EngineArgs a0 = args(sums, max_steps, 0);
     a0.perm = bin ? pending_perm : nullptr;
     pending_perm = nullptr;
-    launch_transfer(0, s, a0);
+    launch_transfer(0, s, a0, sep);
   }
   if (corr_on) launch_cellf(s, args(sums, max_steps, 0));
-  launch_grid(s, grid_args(0), ws, len, 0);
+  launch_grid(s, grid_args(0), ws, len, 0, sep ? 0 : 1);
   if (corr_on) launch_corrector(s, args(sums, max_steps, 0), nullptr);
   for (int k = 1; k < len; ++k) {
-    launch_transfer(1, s, args(sums, max_steps, k & 1));
+    launch_transfer(1, s, args(sums, max_steps, k & 1), sep);
     enqueue_frame(s, frames, step0 + k - 1);  // substep k-1 finished its G2P
     enqueue_state(s, sink, step0 + k - 1);
     if (corr_on) launch_cellf(s, args(sums, max_steps, k & 1));
-    launch_grid(s, grid_args(k), ws, len, k);
+    launch_grid(s, grid_args(k), ws, len, k, sep ? 0 : 1);
     if (corr_on) launch_corrector(s, args(sums, max_steps, k & 1), nullptr);
   }
   launch_transfer(2, s, args(sums, max_steps, len & 1));

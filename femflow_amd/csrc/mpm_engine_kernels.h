@@ -102,10 +102,13 @@ struct EngineArgs {
   uint32_t* list_count;  // their number
   uint32_t* other_count; // the other list's count (cleared by the grid update)
   uint32_t* counters;  // [0], [1] marked-block list counts, [2] sums step base,
-                       // [3] deferred particles, [4] candidate blocks,
-                       // [5] binning generation
+                       // [3], [6] deferred particles (by substep parity; bit 31:
+                       // finished), [4] candidate blocks, [5] binning
+                       // generation, [7] grid-update start ticket
   uint32_t* hard_list;   // deferred particles: index << 2 | plasticity pending | no P2G
-  uint32_t* hard_count;  // == counters + 3
+  uint32_t* hard_count;  // this substep's deferred count (counters[3] or [6] by parity)
+  uint32_t* other_hard;  // the other parity's count (cleared by the grid update)
+  uint32_t* defer_ticket;  // counters[7]: start order of the grid update's workgroups
   const int4* chunks;  // start, count, global block, scene (one per occupied block)
   const uint32_t* keys;  // binned (sorted) cell keys of the current order
   const uint32_t* nchunks;

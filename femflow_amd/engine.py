@@ -148,6 +148,8 @@ class Engine:
                    "run")
         if check_oob:
             n = self.oob_count()
+            if n >= 1 << 40:  # the grid update's wait for the FP64 pass ran out
+                raise RuntimeError("the FP64 pass for deferred particles did not finish in time")
             if n:
                 raise RuntimeError(f"{n} particle stencil(s) left the grid")
 

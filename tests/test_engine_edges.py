@@ -154,6 +154,50 @@ def test_engine_deferred_fp64_path(gpu):
     eng.close()
 
 
+def test_engine_deferred_in_grid_update_matches_separate_pass(gpu):
+    """A plain run finishes the FP64-deferred particles inside the grid update
+    (the first-started workgroup, the others waiting for it); a run that
+    writes every substep's state finishes them in their own launch.  Over 4
+    substeps (both deferred-count parities, graph replay) with ~600
+    near-degenerate particles the two agree to float-atomic ordering, and
+    both defer in every substep."""
+    rng = np.random.default_rng(13)
+    st, pr = O.scene_3d_cubes(n=20000, res=32, seed=14, side=0.25, corners=((0.3, 0.35, 0.3),),
+                              materials=(O.MATERIALS_C2[0],), vel_sigma=0.2)
+    n = len(st["x"])
+    F = st["F"].reshape(n, 3, 3)
+    for p in rng.choice(n, size=600, replace=False):
+        F[p] = _rotation(rng) @ np.diag([1.02, 0.98, 2e-4]) @ _rotation(rng)
+    st["F"] = F.reshape(st["F"].shape)
+    ts = to_torch_state(st, gpu)
+    k = 4
+    eng = engine_for(st, pr, gpu, sort_every=48)
+    fused = engine_roundtrip(eng, ts, k)
+    fb_fused = eng.fallback_count()
+    # the same run again: the captured graph replays
+    again = engine_roundtrip(eng, ts, k)
+    fb_again = eng.fallback_count()
+    eng.upload(ts["x"], ts["v"], ts["F"], ts["C"], ts["Jp"], ts["mass"], ts["mu0"], ts["lam0"], ts["kind"])
+    rec = {f: torch.empty((k,) + tuple(ts[f].shape), device=gpu) for f in ("x", "v", "F", "C", "Jp")}
+    eng.run_states(k, rec["x"], rec["v"], rec["F"], rec["C"], rec["Jp"])
+    fb_sep = eng.fallback_count()
+    torch.cuda.synchronize()
+    _say("deferred fused/again/separate", fb_fused, fb_again, fb_sep)
+    assert min(fb_fused, fb_again, fb_sep) >= 4 * 500, (fb_fused, fb_again, fb_sep)
+    assert abs(fb_fused - fb_sep) <= 0.01 * fb_sep and abs(fb_again - fb_sep) <= 0.01 * fb_sep
+    # the bars of the FP64-oracle check above: C (a velocity gradient, times
+    # inv_dx) shows the float-atomic order of the deferred scatter the most
+    bars = {"x": 1e-5, "v": 1e-5, "F": 1e-5, "C": 1e-4, "Jp": 1e-5}
+    for f in ("x", "v", "F", "C", "Jp"):
+        sep = rec[f][-1]
+        scale = float(sep.abs().max()) + 1e-30
+        for got in (fused[f], again[f]):
+            d = float((got - sep).abs().max()) / scale
+            _say("fused vs separate", f, d)
+            assert d < bars[f], (f, d)
+    eng.close()
+
+
 def test_engine_drift_between_rebins(gpu):
     """Fast particles and rare rebinning: over 24 substeps without a rebin the
     particles travel ~2 cells, so many leave the 8^3 arena of the block they
