@@ -474,6 +474,10 @@ int32_t mpm_substep(mpm_ctx* ctx, mpm_particles* p, const int32_t* scene_ids, in
   if ((rc = mpm_state_download(ctx, p))) return rc;
   int64_t oob = 0;
   if ((rc = ctx->eng.take_counter(0, &oob))) return rc;
+  if (oob >= (int64_t(1) << 40)) {
+    set_error("mpm_substep: the FP64 pass for deferred particles did not finish in time");
+    return MPM_ERR_HIP;
+  }
   if (oob) {
     set_error("mpm_substep: " + std::to_string(oob) + " particle stencils left the grid");
     return MPM_ERR_OOB;

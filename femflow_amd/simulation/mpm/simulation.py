@@ -162,20 +162,23 @@ class MPMSimulation(SimulationBase):
         self._simulate_offline(raise_errors=True)
 
     def _simulate_offline(self, raise_errors=False):
+        """The rollout in chunks of substeps, each writing its per-substep
+        positions to a device frame buffer (simulation.py:125-142 keeps every
+        frame).  Two device buffers alternate: while the engine computes chunk
+        j into one, a copy stream moves chunk j-1 from the other into pinned
+        host memory and the host converts it (float64, divided by
+        tightening_coeff) -- the frame output does not stall the step."""
         self.running = True
         try:
             with torch.cuda.device(self.device):
-                done = 0
-                while done < self.steps:
-                    k = min(self.frames_per_chunk, self.steps - done)
-                    if self.save_displacements:
-                        frames = torch.empty((k, self.n, 3), dtype=torch.float32, device=self.device)
-                        self.engine.run_frames(k, frames)
-                        host = frames.double().cpu().numpy() / self.tightening_coeff
-                        self.displacements.extend(host.reshape(k, -1))
-                    else:
+                if self.save_displacements:
+                    self._rollout_with_frames()
+                else:
+                    done = 0
+                    while done < self.steps:
+                        k = min(self.frames_per_chunk, self.steps - done)
                         self.engine.run(k)
-                    done += k
+                        done += k
             if self.save_displacements:
                 os.makedirs(self.outdir, exist_ok=True)
                 for i, displacement in enumerate(self.displacements):
@@ -187,6 +190,47 @@ class MPMSimulation(SimulationBase):
                 raise
         finally:
             self.running = False
+
+    def _rollout_with_frames(self):
+        n, dev = self.n, self.device
+        # frames per chunk: at most frames_per_chunk, and about 512 MB per buffer
+        fpc = max(1, min(self.frames_per_chunk, (512 << 20) // max(1, n * 12)))
+        cur = torch.cuda.current_stream(dev)
+        copy_stream = torch.cuda.Stream(dev)
+        dbuf, hbuf, copied = [None, None], [None, None], [None, None]
+        pending = None  # (slot, frames) copied out but not yet converted
+
+        def convert(slot, k):
+            copied[slot].synchronize()
+            host = hbuf[slot][:k].numpy().astype(np.float64) / self.tightening_coeff
+            self.displacements.extend(host.reshape(k, -1))
+
+        done, j = 0, 0
+        while done < self.steps:
+            k = min(fpc, self.steps - done)
+            slot = j & 1
+            if dbuf[slot] is None:
+                dbuf[slot] = torch.empty((fpc, n, 3), dtype=torch.float32, device=dev)
+                hbuf[slot] = torch.empty((fpc, n, 3), dtype=torch.float32, pin_memory=True)
+            if copied[slot] is not None:
+                cur.wait_event(copied[slot])  # the copy out of this buffer is done
+            self.engine.run_frames(k, dbuf[slot][:k], check_oob=False)
+            copy_stream.wait_stream(cur)
+            with torch.cuda.stream(copy_stream):
+                hbuf[slot][:k].copy_(dbuf[slot][:k], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(copy_stream)
+            copied[slot] = ev
+            if pending is not None:
+                convert(*pending)  # overlaps this chunk's substeps
+            pending = (slot, k)
+            oob = self.engine.oob_count()
+            if oob:
+                raise RuntimeError(f"{oob} particle stencil(s) left the grid")
+            done += k
+            j += 1
+        if pending is not None:
+            convert(*pending)
 
     # ------------------------------------------------------------ state views
     def _download(self):

@@ -198,7 +198,9 @@ int32_t mpm_state_positions(mpm_ctx* ctx, float* x);
 /* Convenience for the reference's per-call pattern: upload, run, download. */
 int32_t mpm_substep(mpm_ctx* ctx, mpm_particles* p, const int32_t* scene_ids, int32_t nsteps);
 /* Particles whose stencil left their grid since the context was created or
- * this counter was last read (reading resets it).  Synchronises the stream. */
+ * this counter was last read (reading resets it).  Synchronises the stream.
+ * A value of 2^40 or more means a grid update's bounded wait for the FP64
+ * pass over deferred particles ran out (an internal error, not particles). */
 int32_t mpm_ctx_oob_count(mpm_ctx* ctx, int64_t* out);
 /* FP32 polar/SVD evaluations that fell back to FP64 (reading resets it). */
 int32_t mpm_ctx_fallback_count(mpm_ctx* ctx, int64_t* out);
