@@ -42,8 +42,10 @@ def test_batched_scenes_match_oracle_per_scene(gpu):
             O.substep(ref, pr)
         errs = field_errors({k: v[off:off + n] for k, v in out.items()}, ref, pr["inv_dx"])
         print("scene", i, "res", pr["res"], "n", n, errs, flush=True)
-        for k in ("x", "v", "F"):
-            assert errs[k] < 1e-4, (i, errs)
+        # about ten times the error measured after 6 substeps (x <= 2.4e-7,
+        # v <= 8.8e-7, F <= 2.3e-8 over the 12 scenes)
+        for k, b in (("x", 3e-6), ("v", 1e-5), ("F", 5e-7)):
+            assert errs[k] < b, (i, k, errs)
         off += n
     run.close()
 
@@ -116,7 +118,9 @@ def test_full_size_batch_matches_oracle_on_selected_scenes(gpu):
         got = CO.run(ref, pr, steps, nthreads=nt).to_state()
         errs = field_errors({k: v[a:b] for k, v in out.items()}, got, pr["inv_dx"])
         print(f"scene {i} ({why}): n={b - a} res={pr['res']} material={mats[i]}", errs, flush=True)
-        for k in ("x", "v", "F"):
-            assert errs[k] < 1e-4, (i, why, errs)
+        # about ten times the error measured (x <= 1.6e-7, v <= 8.5e-7,
+        # F <= 2.1e-8 on the selected scenes)
+        for k, bar in (("x", 3e-6), ("v", 1e-5), ("F", 5e-7)):
+            assert errs[k] < bar, (i, why, k, errs)
     assert run.engine.oob_count() == 0
     run.close()

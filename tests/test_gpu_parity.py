@@ -150,8 +150,10 @@ def test_engine_reupload_changes_count(gpu):
             O.substep(ref, pr)
         errs = field_errors(to_numpy_state(out), ref, pr["inv_dx"])
         _say("reupload", n, errs)
-        for k in ("x", "v", "F"):
-            assert errs[k] < 1e-4, (n, errs)
+        # about ten times the error measured after 6 substeps (x 2e-7,
+        # v 5e-6, F 2.3e-7)
+        for k, b in (("x", 5e-6), ("v", 5e-5), ("F", 5e-6)):
+            assert errs[k] < b, (n, k, errs)
         assert bool((guard[n:] == 7.0).all()), "download wrote past the current particle count"
     eng.close()
 
@@ -289,9 +291,12 @@ def test_engine_fused_multistep(gpu, sort_every, use_graph, chunk, spread):
         O.substep(ref, pr)
     errs = field_errors(out, ref, pr["inv_dx"])
     _say("engine fused", sort_every, use_graph, chunk, spread, errs)
-    for k in ("x", "v", "F"):
-        assert errs[k] < 1e-4, errs
-    assert errs["Jp"] < 1e-4, errs
+    # about ten times the accumulated FP32 error measured after 10 substeps
+    # (x 3.3e-7, v 1.8e-6, F 2e-8, C 4e-7, Jp 1e-9, the same for every launch
+    # shape here)
+    bars = {"x": 5e-6, "v": 2e-5, "F": 1e-6, "C": 5e-6, "Jp": 1e-7}
+    for k, b in bars.items():
+        assert errs[k] < b, (k, errs)
     eng.close()
 
 
@@ -375,8 +380,10 @@ def test_engine_batched_scenes_match_single(gpu):
         got = {k: v[off:off + n] for k, v in out.items()}
         errs = field_errors(got, ref, pr["inv_dx"])
         _say("batched scene", pr["res"], errs)
-        for k in ("x", "v", "F"):
-            assert errs[k] < 1e-4, errs
+        # about ten times the error measured after 5 substeps (x 1.7e-7,
+        # v 4.2e-6, F 2.2e-7)
+        for k, b in (("x", 5e-6), ("v", 5e-5), ("F", 5e-6)):
+            assert errs[k] < b, (k, errs)
         off += n
     eng.close()
 
