@@ -11,8 +11,9 @@ namespace mpm {
 size_t radix_workspace_bytes(int64_t n);
 int radix_passes(uint32_t max_key);
 // Chunk list of a binning: occupied block b (bocc[b] != 0, particles
-// bstart[b] .. bend[b]) gets ceil(count / chunk) chunks (start, count, b,
-// block_scene[b]), numbered after those of the blocks before it; bfirst[b]
+// bstart[b] .. bend[b]) gets ceil(count / chunk) chunks of nearly equal size
+// (start, count, b, block_scene[b]), numbered after those of the blocks
+// before it; bfirst[b]
 // and bcount[b] are its first chunk and chunk count (0 for empty blocks);
 // *nchunks = the total.  part: chunk_list_part_words(nblocks) words.
 size_t chunk_list_part_words(int64_t nblocks);

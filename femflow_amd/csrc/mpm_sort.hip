@@ -179,8 +179,8 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* s
 
 // ------------------------------------------------------- chunk list ----
 // Chunk list of the new binning, numbered over blocks: block b holds
-// ceil(count_b / chunk) chunks, numbered after those of the blocks before
-// it.  Two launches over tiles of SCAN_THREADS blocks, one block per thread
+// ceil(count_b / chunk) chunks of nearly equal size, numbered after those
+// of the blocks before it.  Two launches over tiles of SCAN_THREADS blocks, one block per thread
 // (coalesced reads, a few hundred tiles for a 128^3 grid): per-tile chunk
 // counts, then each tile adds the counts of the tiles before it, scans its
 // own blocks and writes their chunks, first chunk and chunk count.
@@ -223,10 +223,11 @@ __global__ __launch_bounds__(SCAN_THREADS) void block_chunks_kernel(
     if (c) {
       const uint32_t p0 = bstart[b], cnt = bend[b] - p0;
       const int sc = block_scene[b];
+      // c chunks of (nearly) equal size rather than full ones and a
+      // remainder: less spread in per-chunk work, a shorter tail
       for (uint32_t j = 0; j < c; ++j) {
-        const uint32_t left = cnt - j * (uint32_t)chunk;
-        chunks[run + j] = make_int4((int)(p0 + j * (uint32_t)chunk),
-                                    (int)(left < (uint32_t)chunk ? left : (uint32_t)chunk), (int)b, sc);
+        const uint32_t s0 = (uint32_t)(((uint64_t)cnt * j) / c), s1 = (uint32_t)(((uint64_t)cnt * (j + 1)) / c);
+        chunks[run + j] = make_int4((int)(p0 + s0), (int)(s1 - s0), (int)b, sc);
       }
     }
   }
