@@ -59,6 +59,9 @@ struct Engine {
   int nmat = 1;
   int chunk = 512, sort_every = 8, grid_mult = 4;
   int spread = 9;  // scatter start offsets per cell (3 or 9, see transfer_kernel)
+  // fused passes take chunks from per-XCD ticket queues (set for engines of
+  // 8 or more scenes, whose many uneven chunks the queue balances)
+  bool dyn_sched = false;
   bool use_graph = true, timing = false;
   // substeps run since the particles were last binned; -1: not binned (after
   // an upload or a rebin request), so the next segment rebins first

@@ -417,7 +417,12 @@ __device__ __forceinline__ int round_fixed(float w, float q) {
 // runtime row index (a few VALU per slab).  Measured at 2^19 particles:
 // equal at 16 particles per cell, 25% / 34% faster at 128 / 1,157, and 1.5%
 // faster on the three-cube 2^20 scene (~21 per cell), so 9 is the default.
-template <int MODE, int P, int SPREAD>
+// DYN: workgroups take their chunks from a per-XCD ticket queue instead of
+// a fixed interleave.  With tens of thousands of uneven chunks (many scenes
+// of different densities) the queue balances the tail; with a few thousand
+// the 8 contended counters cost more than they save, so only the fused pass
+// of multi-scene engines uses it.
+template <int MODE, int P, int SPREAD, bool DYN = false>
 __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   static_assert(SPREAD == 3 || SPREAD == 9, "scatter spread is 3 or 9");
   // The velocity arena (read by the G2P half) and the accumulation arena
@@ -460,7 +465,17 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
   // dependent load fewer ahead of each chunk's velocity arena)
   const bool one_scene = a.nscenes == 1;
   const SceneDev sc0 = a.scenes[0];
-  for (uint32_t c = lo + l; c < hi; c += L) {
+  __shared__ uint32_t next_s;
+  uint32_t* const queue = a.counters + 8 + xcd;  // this XCD's ticket counter
+  if constexpr (DYN) {
+    if (t == 0) next_s = lo + atomicAdd(queue, 1u);
+    __syncthreads();
+  }
+  for (uint32_t c = DYN ? next_s : lo + (uint32_t)l; c < hi;) {
+    uint32_t nxt = 0u;
+    if constexpr (DYN) {
+      if (t == 0) nxt = lo + atomicAdd(queue, 1u);  // in flight while this chunk runs
+    }
     const int4 ch = a.chunks[c];
     const SceneDev sc = one_scene ? sc0 : a.scenes[ch.w];
     const int local = ch.z - sc.block_base;
@@ -729,7 +744,15 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         }
       }
     }
+    if constexpr (DYN) {
+      if (t == 0) next_s = nxt;
+    }
     __syncthreads();
+    if constexpr (DYN) {
+      c = next_s;
+    } else {
+      c += (uint32_t)L;
+    }
   }
   __syncthreads();
   if (t == 0) {
@@ -841,6 +864,7 @@ __global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums,
     counters[3] = 0u;
     counters[6] = 0u;
     counters[7] = 0u;
+    for (int x = 0; x < 8; ++x) counters[8 + x] = 0u;
     counters[3] = 0u;
     if (with_sums) counters[2] += (uint32_t)len;
     if (bin) {
@@ -908,6 +932,7 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *a.other_count = 0u;  // the list the next substep appends to
     *a.other_hard = 0u;   // the deferred count the next substep appends to
+    for (int x = 0; x < 8; ++x) a.counters[8 + x] = 0u;  // the next transfer's chunk queues
   }
   const int t = threadIdx.x;
   const int wv = t >> 6, inner = t & 63;
@@ -1363,6 +1388,7 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   }
   total_blocks = blocks;
   scenes_host = hs;
+  dyn_sched = nscenes >= 8;
   int rc;
   const int64_t n = ((n_max > 0 ? n_max : 1) + PTILE - 1) / PTILE * PTILE;
   if ((rc = alloc((void**)&d_scenes, sizeof(SceneDev) * nscenes))) return rc;
@@ -1564,7 +1590,9 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a, bool 
     else                                                                                             \
       MPM_LAUNCH_K((transfer_kernel<M, PV, 3>));                                                     \
   } while (0)
-  if (chunk == 256) {
+  if (dyn_sched && mode == 1 && chunk == 512 && spread == 9) {
+    MPM_LAUNCH_K((transfer_kernel<1, 2, 9, true>));
+  } else if (chunk == 256) {
     if (mode == 0) MPM_LAUNCH_T(0, 1);
     else if (mode == 1) MPM_LAUNCH_T(1, 1);
     else MPM_LAUNCH_T(2, 1);

@@ -104,7 +104,8 @@ struct EngineArgs {
   uint32_t* counters;  // [0], [1] marked-block list counts, [2] sums step base,
                        // [3], [6] deferred particles (by substep parity; bit 31:
                        // finished), [4] candidate blocks, [5] binning
-                       // generation, [7] grid-update start ticket
+                       // generation, [7] grid-update start ticket,
+                       // [8..15] per-XCD chunk queues of the fused pass
   uint32_t* hard_list;   // deferred particles: index << 2 | plasticity pending | no P2G
   uint32_t* hard_count;  // this substep's deferred count (counters[3] or [6] by parity)
   uint32_t* other_hard;  // the other parity's count (cleared by the grid update)
