@@ -854,9 +854,9 @@ __global__ __launch_bounds__(ENG_THREADS) void deferred_kernel(EngineArgs a) {
 }
 
 // Start of a segment: clear both marked-block list counters, both deferred
-// counts and the grid update's start ticket, advance the per-substep sums
-// index by the segment length, and, when
-// the segment rebins, start a new candidate list (count 0, next generation).
+// counts, the grid update's start ticket and the chunk queues, advance the
+// per-substep sums index by the segment length, and, when the segment
+// rebins, start a new candidate list (count 0, next generation).
 __global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums, int bin) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     counters[0] = 0u;
@@ -865,7 +865,6 @@ __global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums,
     counters[6] = 0u;
     counters[7] = 0u;
     for (int x = 0; x < 8; ++x) counters[8 + x] = 0u;
-    counters[3] = 0u;
     if (with_sums) counters[2] += (uint32_t)len;
     if (bin) {
       counters[4] = 0u;
@@ -1542,7 +1541,9 @@ int Engine::download(mpm_particles* p) {
   return MPM_OK;
 }
 
-// Keys -> stable radix sort -> physical reorder -> chunk list.
+// Keys -> stable radix sort -> block ranges -> chunk list -> candidate blocks
+// and their neighbour table.  The particles themselves move into the new
+// order in the segment's first P2G (through pending_perm).
 void Engine::enqueue_bin(hipStream_t s) {
   if (n <= 0) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_nchunks, (int64_t)1, 0u);

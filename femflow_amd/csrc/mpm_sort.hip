@@ -1,8 +1,8 @@
 // Cell binning: key generation, a stable LSD radix sort on 32-bit keys
 // (8-bit digits; three launches per pass: tile histograms, a per-digit scan
-// over tiles, the stable scatter), and the chunk numbering (a two-launch
-// scan of chunk-start flags) from which the engine builds its per-block work
-// items for the transfer kernels.
+// over tiles, the stable scatter), and the chunk list (a two-launch scan of
+// per-block chunk counts) that gives the engine its work items for the
+// transfer kernels.
 //
 // Stability: inside a tile each wave owns a contiguous run of keys and walks
 // it in index order; ranks among equal digits come from a 64-lane match mask
