@@ -57,7 +57,7 @@ struct Engine {
   int device = 0, num_cus = 256;
   int64_t n_max = 0, n = 0, total_blocks = 0;
   int nmat = 1;
-  int chunk = 512, sort_every = 8, grid_mult = 4;
+  int chunk = 512, sort_every = 48, grid_mult = 8;  // the measured headline optimum
   int spread = 9;  // scatter start offsets per cell (3 or 9, see transfer_kernel)
   // fused passes take chunks from per-XCD ticket queues (set for engines of
   // 8 or more scenes, whose many uneven chunks the queue balances)

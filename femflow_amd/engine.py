@@ -84,7 +84,7 @@ class Engine:
     def _sync_stream(self):
         _lib.check(self._L.mpm_ctx_set_stream(self._h, self._stream()), "set_stream")
 
-    def tune(self, chunk: int = 512, sort_every: int = 8, grid_mult: int = 4, use_graph: bool = True,
+    def tune(self, chunk: int = 512, sort_every: int = 48, grid_mult: int = 8, use_graph: bool = True,
              spread: int = 9):
         """chunk: binned particles per work item (256, 512, 1024); sort_every:
         substeps between rebinning; grid_mult: persistent workgroups per CU;

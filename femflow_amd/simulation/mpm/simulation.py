@@ -70,7 +70,7 @@ class MPMSimulation(SimulationBase):
                  volume: float, force: float, gyroid_E: float, collider_E: float, gyroid_v: float,
                  collider_v: float, hardening: float, grid_res: int, tightening_coeff: float,
                  save_displacements=True, device=None, frames_per_chunk: int = 256,
-                 sort_every: int = 8):
+                 sort_every: int = 48):
         super().__init__()
         self.outdir = outdir
         self.save_displacements = save_displacements

@@ -148,7 +148,8 @@ int32_t mpm_ctx_destroy(mpm_ctx* ctx);
 int32_t mpm_ctx_set_stream(mpm_ctx* ctx, void* stream);
 /* Tunables: chunk = particles per work item (256, 512 or 1024); sort_every =
  * substeps between rebinning (>= 1); grid_mult = persistent workgroups per CU;
- * use_graph = capture and replay the substep sequence as a hipGraph. */
+ * use_graph = capture and replay the substep sequence as a hipGraph.  A new
+ * context starts at chunk 512, sort_every 48, grid_mult 8, graphs on. */
 int32_t mpm_ctx_tune(mpm_ctx* ctx, int32_t chunk, int32_t sort_every, int32_t grid_mult,
                      int32_t use_graph);
 /* Scatter spread (3 or 9, default 9): the number of different starting nodes
