@@ -565,6 +565,10 @@ def main():
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "traffic_source": "profiles/g2p2g_c2_1M_128_traffic.json (rocprofv3 PMC run of this workload)",
+            # the HBM rate the kernel actually sustains: measured bytes (the
+            # fused pass neither reads nor writes v and C, which the next G2P
+            # recomputes) over the same mean duration
+            "traffic_gbs": (traffic / avg_s / 1e9) if traffic else None,
             "kernel": dom, "kernel_ms": avg_s * 1e3,
             "alg_bytes_per_launch": alg_bytes, "active_nodes": active_nodes,
             "grid_blocks_processed": grid_blocks,

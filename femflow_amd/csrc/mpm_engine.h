@@ -62,6 +62,9 @@ struct Engine {
   // fused passes take chunks from per-XCD ticket queues (set for engines of
   // 8 or more scenes, whose many uneven chunks the queue balances)
   bool dyn_sched = false;
+  // set while a segment with per-substep state output is enqueued: the fused
+  // pass then writes v and C every substep
+  bool store_vc = false;
   bool use_graph = true, timing = false;
   // substeps run since the particles were last binned; -1: not binned (after
   // an upload or a rebin request), so the next segment rebins first

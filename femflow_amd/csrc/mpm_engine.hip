@@ -561,10 +561,17 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
           gather<false>(vel_s, 0, a.vel, sc, base, w, fx, v, C);
         const bool ok = advance(sc, mp, x, v, C, F, Jp);
         storek<3>(a.s.x, p, x);
-        storek<3>(a.s.v, p, v);
-        storek<9>(a.s.C, p, C);
         storek<9>(a.s.F, p, F);
         a.s.Jp[p] = Jp;
+        // v and C are this G2P's output.  In the fused pass the P2G below
+        // consumes them from registers and the next G2P recomputes both, so
+        // they go to memory only where they are read: after the last G2P of
+        // a segment, for per-substep state output, and for particles the
+        // FP64 pass finishes.
+        if (MODE == 2 || a.store_vc || !ok) {
+          storek<3>(a.s.v, p, v);
+          storek<9>(a.s.C, p, C);
+        }
         if (!ok) {
           defer(a, p, MODE == 2 ? 3u : 1u);
           continue;
@@ -579,6 +586,10 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
         P2GTerms tm;
         if (!p2g_terms(sc, mp, pmass, pmu0, plam0, Jp, F, C, v, fx, tm)) {
+          if (MODE == 1 && !a.store_vc) {  // the FP64 pass reads them
+            storek<3>(a.s.v, p, v);
+            storek<9>(a.s.C, p, C);
+          }
           defer(a, p, 0u);
           continue;
         }
@@ -1487,6 +1498,7 @@ EngineArgs Engine::args(double* sums, int max_steps, int sel) const {
   a.list_cap = (uint32_t)total_blocks;
   a.oob = d_oob;
   a.hard_list = d_hard;
+  a.store_vc = store_vc ? 1 : 0;
   a.hard_count = d_counters + (sel ? 6 : 3);
   a.other_hard = d_counters + (sel ? 3 : 6);
   a.defer_ticket = d_counters + 7;
@@ -1675,8 +1687,10 @@ void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int
     return a;
   };
   // the deferred FP64 particles are finished by the grid update unless the
-  // particles are read in between (cell-F sums, per-substep state output)
+  // particles are read in between (cell-F sums, per-substep state output);
+  // per-substep state output also needs each substep's v and C in memory
   const bool sep = corr_on || sink != nullptr;
+  store_vc = sink != nullptr;
   {
     EngineArgs a0 = args(sums, max_steps, 0);
     a0.perm = bin ? pending_perm : nullptr;
@@ -1697,6 +1711,7 @@ void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int
   launch_transfer(2, s, args(sums, max_steps, len & 1));
   enqueue_frame(s, frames, step0 + len - 1);
   enqueue_state(s, sink, step0 + len - 1);
+  store_vc = false;
 }
 
 int Engine::run(int nsteps, double* sums, float* frames, const StateSink* sink) {

@@ -106,6 +106,7 @@ struct EngineArgs {
                        // finished), [4] candidate blocks, [5] binning
                        // generation, [7] grid-update start ticket,
                        // [8..15] per-XCD chunk queues of the fused pass
+  int store_vc;          // fused pass: write v and C every substep (per-substep state output)
   uint32_t* hard_list;   // deferred particles: index << 2 | plasticity pending | no P2G
   uint32_t* hard_count;  // this substep's deferred count (counters[3] or [6] by parity)
   uint32_t* other_hard;  // the other parity's count (cleared by the grid update)
