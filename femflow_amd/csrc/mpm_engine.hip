@@ -509,7 +509,11 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
       uint32_t attr;
       if (MODE == 0 && a.perm) {
         // binned slot p takes the particle of slot perm[p] of the previous
-        // order: every field is copied here, before any early exit
+        // order: every field the fused pass reads is copied here, before any
+        // early exit.  v and C are read by the P2G below from registers and
+        // recomputed by the next G2P, so the new slot gets them only where
+        // they are read before that (FP64-deferred particles, per-substep
+        // state output) or kept (particles that left the grid).
         const int64_t ps = a.perm[p];
         loadk<3>(a.src.x, ps, x);
         loadk<9>(a.src.F, ps, F);
@@ -523,8 +527,10 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         const uint32_t ord = a.src.order[ps];
         storek<3>(a.s.x, p, x);
         storek<9>(a.s.F, p, F);
-        storek<3>(a.s.v, p, v);
-        storek<9>(a.s.C, p, C);
+        if (a.store_vc) {
+          storek<3>(a.s.v, p, v);
+          storek<9>(a.s.C, p, C);
+        }
         a.s.Jp[p] = Jp;
         a.s.attr[p] = attr;
         a.s.mass[p] = pmass;
@@ -577,8 +583,14 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
           continue;
         }
       }
+      // MODE 0 after a rebin: v and C not yet in the new slot
+      const bool vc_unstored = MODE == 0 && a.perm && !a.store_vc;
       if constexpr (MODE != 2) {
         if (!locate3(x, sc, base, fx)) {
+          if (vc_unstored) {
+            storek<3>(a.s.v, p, v);
+            storek<9>(a.s.C, p, C);
+          }
           atomicAdd(&wg_oob, 1u);
           continue;
         }
@@ -586,7 +598,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
         P2GTerms tm;
         if (!p2g_terms(sc, mp, pmass, pmu0, plam0, Jp, F, C, v, fx, tm)) {
-          if (MODE == 1 && !a.store_vc) {  // the FP64 pass reads them
+          if ((MODE == 1 && !a.store_vc) || vc_unstored) {  // the FP64 pass reads them
             storek<3>(a.s.v, p, v);
             storek<9>(a.s.C, p, C);
           }
@@ -729,7 +741,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
       __syncthreads();
       const double inv[4] = {cs.inv[0], cs.inv[1], cs.inv[2], cs.inv[3]};
       const bool staged = c < a.stage_cap;
-      float4* stage = a.stage + (size_t)c * STAGE_NODES;
+      float4* stage = a.stage + (size_t)c * STAGE_REC;
 #pragma unroll 1
       for (int e = t; e < ARENA_NODES; e += ENG_THREADS) {
         const int e0 = e >> 6, e1 = (e >> 3) & 7, e2 = e & 7;
@@ -1016,10 +1028,10 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
     float4 sv[3][8];
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn) {
-      const float4* src = a.stage + (size_t)c0[sn] * STAGE_NODES + ai[sn];
+      const float4* src = a.stage + (size_t)c0[sn] * STAGE_REC + ai[sn];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
-        sv[c][sn] = cn[sn] > (uint32_t)c ? src[(size_t)c * STAGE_NODES] : make_float4(0.f, 0.f, 0.f, 0.f);
+        sv[c][sn] = cn[sn] > (uint32_t)c ? src[(size_t)c * STAGE_REC] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn) {
@@ -1044,7 +1056,7 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       for (int sn = 0; sn < 8; ++sn)
 #pragma unroll
         for (int u = 0; u < 2; ++u)
-          v2[sn][u] = c + u < cn[sn] ? a.stage[(size_t)(c0[sn] + c + u) * STAGE_NODES + ai[sn]]
+          v2[sn][u] = c + u < cn[sn] ? a.stage[(size_t)(c0[sn] + c + u) * STAGE_REC + ai[sn]]
                                      : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
       for (int sn = 0; sn < 8; ++sn)
@@ -1443,7 +1455,7 @@ int Engine::init(int64_t n_max_, int nscenes, const mpm_scene* sc, int nmat_, co
   const int64_t cap_chunks = (max_chunks < n + 1 ? max_chunks : n + 1);
   const int64_t want = 2 * ((n + 255) / 256) + 8192;
   stage_cap = (uint32_t)(want < cap_chunks ? want : cap_chunks);
-  if ((rc = alloc((void**)&d_stage, sizeof(float4) * STAGE_NODES * (size_t)stage_cap))) return rc;
+  if ((rc = alloc((void**)&d_stage, sizeof(float4) * STAGE_REC * (size_t)stage_cap))) return rc;
   if ((rc = alloc((void**)&d_bfirst, sizeof(uint32_t) * blocks))) return rc;
   if ((rc = alloc((void**)&d_bcount, sizeof(uint32_t) * blocks))) return rc;
   HIPCHK(hipMemset(d_bcount, 0, sizeof(uint32_t) * blocks));

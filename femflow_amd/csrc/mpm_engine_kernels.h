@@ -24,26 +24,32 @@ constexpr int ARENA_NODES = ARENA * ARENA * ARENA;
 // particles that drifted out of the block since binning and goes through
 // global atomics.
 constexpr int STAGE = 7;
-constexpr int STAGE_NODES = STAGE * STAGE * STAGE;
 
-// Slot of staged node (x0, x1, x2) (each 0..6) in its chunk's staging
-// record.  Per axis the seven nodes split by the block they belong to:
-// node 0 lies in the block before the chunk's own (part 0), nodes 1..4 in
-// its own (part 1), nodes 5..6 in the block after (part 2).  The record
-// stores the 27 part boxes one after another (part-major, sizes 1, 4, 2 per
-// axis), each in row-major order, so the grid update of one block reads
-// each source chunk's contribution to it as one contiguous run of slots in
-// its lanes' order.
+// Staging record of one chunk.  Per axis the seven staged nodes split by
+// the block they belong to: node 0 lies in the block before the chunk's own
+// (part 0), nodes 1..4 in its own (part 1), nodes 5..6 in the block after
+// (part 2).  The record is cut into 27 part boxes like an 8x8x8 array cut at
+// 2 and 6 per axis (box extents 2, 4, 2: never below 8 slots, always a
+// multiple of 8), in part-major order; a box holds its part's nodes
+// (extents 1, 4, 2) row-major in its first slots.  So the grid update of
+// one block reads each source chunk's contribution to it as one contiguous
+// run of slots in its lanes' order, and every run starts on a 128-byte line
+// (8 slots): it fetches no line of another box.  512 slots per record for
+// 343 nodes; the unused slots are never written or read.
+constexpr int STAGE_REC = 512;
+
+// Slot of staged node (x0, x1, x2) (each 0..6) in its chunk's record.  The
+// per-part constants are read from nibbles (part p at bits 4p): first node
+// s (0, 1, 5), extent z (1, 4, 2), first box coordinate t (0, 2, 6), box
+// extent e (2, 4, 2).
 __device__ __forceinline__ int stage_slot(int x0, int x1, int x2) {
   const int p0 = x0 == 0 ? 0 : (x0 < 5 ? 1 : 2), p1 = x1 == 0 ? 0 : (x1 < 5 ? 1 : 2),
             p2 = x2 == 0 ? 0 : (x2 < 5 ? 1 : 2);
-  // part start (0, 1, 5) and size (1, 4, 2)
-  const int s0 = p0 == 0 ? 0 : (p0 == 1 ? 1 : 5), s1 = p1 == 0 ? 0 : (p1 == 1 ? 1 : 5),
-            s2 = p2 == 0 ? 0 : (p2 == 1 ? 1 : 5);
-  const int z0 = p0 == 0 ? 1 : (p0 == 1 ? 4 : 2), z1 = p1 == 0 ? 1 : (p1 == 1 ? 4 : 2),
-            z2 = p2 == 0 ? 1 : (p2 == 1 ? 4 : 2);
-  const int base = s0 * (STAGE * STAGE) + z0 * s1 * STAGE + z0 * z1 * s2;
-  return base + ((x0 - s0) * z1 + (x1 - s1)) * z2 + (x2 - s2);
+  const int s0 = (0x510 >> (4 * p0)) & 15, s1 = (0x510 >> (4 * p1)) & 15, s2 = (0x510 >> (4 * p2)) & 15;
+  const int z1 = (0x241 >> (4 * p1)) & 15, z2 = (0x241 >> (4 * p2)) & 15;
+  const int t0 = (0x620 >> (4 * p0)) & 15, t1 = (0x620 >> (4 * p1)) & 15, t2 = (0x620 >> (4 * p2)) & 15;
+  const int e0 = (0x242 >> (4 * p0)) & 15, e1 = (0x242 >> (4 * p1)) & 15;
+  return t0 * 64 + e0 * t1 * 8 + e0 * e1 * t2 + ((x0 - s0) * z1 + (x1 - s1)) * z2 + (x2 - s2);
 }
 constexpr int MAX_SCENES = 4096;
 

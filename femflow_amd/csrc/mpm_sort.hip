@@ -49,11 +49,19 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint32_t* __restr
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   const unsigned lane = __lane_id();
-#pragma unroll 4
+  // every key of the thread in flight at once: one memory round trip per
+  // tile rather than one per few rounds
+  uint32_t k[RS_ITEMS];
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int64_t idx = base + (int64_t)r * RS_THREADS + t;
+    k[r] = idx < n ? keys[idx] : 0u;
+  }
+#pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
     const int64_t idx = base + (int64_t)r * RS_THREADS + t;
     const bool valid = idx < n;
-    const uint32_t d = valid ? (keys[idx] >> shift) & 255u : 0u;
+    const uint32_t d = valid ? (k[r] >> shift) & 255u : 0u;
     const uint64_t vm = __ballot(valid);
     const uint64_t m = match_digit(d, vm);
     if (valid && lane == 63u - (unsigned)__clzll(m)) atomicAdd(&h[d], (uint32_t)__popcll(m));
@@ -121,14 +129,21 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
   goff[t] = excl_scan_256(dtot[t], wt) + offs[(int64_t)t * ntiles + blockIdx.x];
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * 64 * RS_ITEMS;
-  uint32_t k[RS_ITEMS];
+  uint32_t k[RS_ITEMS], val[RS_ITEMS];
   uint32_t rank[RS_ITEMS];
   const uint64_t lt = lanemask_lt();
+  // keys and values of the wave's run loaded together, ahead of the ranking
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
     const int64_t idx = base + r * 64 + lane;
     const bool valid = idx < n;
     k[r] = valid ? kin[idx] : 0u;
+    val[r] = iota_values ? (uint32_t)idx : (valid ? vin[idx] : 0u);
+  }
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int64_t idx = base + r * 64 + lane;
+    const bool valid = idx < n;
     const uint32_t d = (k[r] >> shift) & 255u;
     const uint64_t vm = __ballot(valid);
     const uint64_t m = match_digit(d, vm);
@@ -155,7 +170,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
       const uint32_t d = (k[r] >> shift) & 255u;
       const uint32_t dst = goff[d] + cnt[w][d] + rank[r];
       kout[dst] = k[r];
-      vout[dst] = iota_values ? (uint32_t)idx : vin[idx];
+      vout[dst] = val[r];
     }
   }
 }
