@@ -361,8 +361,7 @@ int Engine::corrector_sample(uint16_t* X, uint32_t* nodes, float* vel, int64_t m
     set_error("mpm_corrector_sample: stream setup failed");
     return MPM_ERR_HIP;
   }
-  begin_segment(s, 1, 0, 1);
-  enqueue_bin(s);
+  enqueue_bin(s, 1, 0);  // also starts the one-substep segment
   since_bin = 0;  // freshly binned: the next run continues from this order
   const EngineArgs a = args(nullptr, 0, 0);
   {

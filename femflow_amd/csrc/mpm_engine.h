@@ -124,7 +124,8 @@ struct Engine {
   EngineArgs args(double* sums, int max_steps, int sel) const;
   int upload(const mpm_particles* p, const int32_t* scene_ids);
   int download(mpm_particles* p);
-  void enqueue_bin(hipStream_t s);
+  // rebin; also starts the segment of len substeps (counters, sums index)
+  void enqueue_bin(hipStream_t s, int len, int with_sums);
   void enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int max_steps, float* frames = nullptr,
                        int64_t step0 = 0, const StateSink* sink = nullptr);
   // destroy every captured graph (after anything they baked in changes)
@@ -139,7 +140,6 @@ struct Engine {
   // the following launch_grid(..., deferred = 1) finishes them
   void launch_transfer(int mode, hipStream_t s, const EngineArgs& a, bool separate_deferred = true);
   void launch_grid(hipStream_t s, const EngineArgs& a, int with_sums, int len, int k, int deferred = 0);
-  void begin_segment(hipStream_t s, int len, int with_sums, int bin);
   void fill_u32(hipStream_t s, uint32_t* p, int64_t count, uint32_t v);
   void launch_cellf(hipStream_t s, const EngineArgs& a);
   // corrector after a grid update; ex != null: write features instead (sampling)

@@ -11,6 +11,7 @@
 #include "mpm_engine.h"
 #include "mpm_engine_kernels.h"
 #include "mpm_sort.h"
+#include "mpm_sort_dev.h"
 
 namespace mpm {
 
@@ -880,20 +881,22 @@ __global__ __launch_bounds__(ENG_THREADS) void deferred_kernel(EngineArgs a) {
 // counts, the grid update's start ticket and the chunk queues, advance the
 // per-substep sums index by the segment length, and, when the segment
 // rebins, start a new candidate list (count 0, next generation).
-__global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums, int bin) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    counters[0] = 0u;
-    counters[1] = 0u;
-    counters[3] = 0u;
-    counters[6] = 0u;
-    counters[7] = 0u;
-    for (int x = 0; x < 8; ++x) counters[8 + x] = 0u;
-    if (with_sums) counters[2] += (uint32_t)len;
-    if (bin) {
-      counters[4] = 0u;
-      counters[5] += 1u;
-    }
+__device__ __forceinline__ void segment_begin(uint32_t* counters, int len, int with_sums, int bin) {
+  counters[0] = 0u;
+  counters[1] = 0u;
+  counters[3] = 0u;
+  counters[6] = 0u;
+  counters[7] = 0u;
+  for (int x = 0; x < 8; ++x) counters[8 + x] = 0u;
+  if (with_sums) counters[2] += (uint32_t)len;
+  if (bin) {
+    counters[4] = 0u;
+    counters[5] += 1u;
   }
+}
+
+__global__ void segment_begin_kernel(uint32_t* counters, int len, int with_sums, int bin) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) segment_begin(counters, len, with_sums, bin);
 }
 
 // ------------------------------------------------------------ grid update ----
@@ -948,8 +951,8 @@ __device__ __forceinline__ void grid_deferred(const EngineArgs& a) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, int with_sums, int len,
-                                                                  int k, int deferred) {
+__global__ __launch_bounds__(ENG_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void grid_update_kernel(
+    EngineArgs a, int with_sums, int len, int k, int deferred) {
   __shared__ double red[ENG_THREADS / 64][8];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     *a.other_count = 0u;  // the list the next substep appends to
@@ -1022,51 +1025,46 @@ __global__ __launch_bounds__(ENG_THREADS) void grid_update_kernel(EngineArgs a, 
       ai[sn] = unstaged ? 0 : stage_slot(x0, x1, x2);
       if (unstaged) cn[sn] = 0u;
     }
-    // up to three chunks per source are loaded in one round trip (a block
-    // of ~20 particles per cell has three 512-particle chunks) and summed
-    // source-major, chunk-minor
-    float4 sv[3][8];
+    // the first two chunks of every source are loaded in one round trip
+    // and summed source-major, chunk-minor; further chunks (a block of ~20
+    // particles per cell has three) one per round, all sources together, so
+    // a block's dependent rounds follow its densest source rather than the
+    // sum over sources.  Two per source rather than three keeps the kernel
+    // at 168 registers, three waves per SIMD: every listed block of a
+    // 2^20-particle scene then has a resident wave from the start.
+    float4 sv[2][8];
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn) {
       const float4* src = a.stage + (size_t)c0[sn] * STAGE_REC + ai[sn];
 #pragma unroll
-      for (int c = 0; c < 3; ++c)
+      for (int c = 0; c < 2; ++c)
         sv[c][sn] = cn[sn] > (uint32_t)c ? src[(size_t)c * STAGE_REC] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
+      for (int c = 0; c < 2; ++c) {
         q.x += sv[c][sn].x;
         q.y += sv[c][sn].y;
         q.z += sv[c][sn].z;
         q.w += sv[c][sn].w;
       }
     }
-    // dense sources (more than three chunks): each round loads the next two
-    // chunks of every source together, so a block's dependent rounds follow
-    // its densest source rather than the sum over sources; the sums keep a
-    // fixed round-major order
     uint32_t cmax = 0;
 #pragma unroll
     for (int sn = 0; sn < 8; ++sn) cmax = cn[sn] > cmax ? cn[sn] : cmax;
-    for (uint32_t c = 3; c < cmax; c += 2) {
-      float4 v2[8][2];
+    for (uint32_t c = 2; c < cmax; ++c) {
+      float4 v2[8];
 #pragma unroll
       for (int sn = 0; sn < 8; ++sn)
+        v2[sn] = c < cn[sn] ? a.stage[(size_t)(c0[sn] + c) * STAGE_REC + ai[sn]] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
-          v2[sn][u] = c + u < cn[sn] ? a.stage[(size_t)(c0[sn] + c + u) * STAGE_REC + ai[sn]]
-                                     : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-      for (int sn = 0; sn < 8; ++sn)
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-          q.x += v2[sn][u].x;
-          q.y += v2[sn][u].y;
-          q.z += v2[sn][u].z;
-          q.w += v2[sn][u].w;
-        }
+      for (int sn = 0; sn < 8; ++sn) {
+        q.x += v2[sn].x;
+        q.y += v2[sn].y;
+        q.z += v2[sn].z;
+        q.w += v2[sn].w;
+      }
     }
     if (a.pm) a.pm[addr] = q;
     const float m = q.w;
@@ -1152,18 +1150,49 @@ __device__ __forceinline__ int cell_of(float x, float inv_dx, int res) {
   return b < 0 ? 0 : (b > res ? res : b);
 }
 
-__global__ void engine_keys_kernel(EngineArgs a, int64_t n, uint32_t* __restrict__ keys,
-                                   uint32_t* __restrict__ bocc, int64_t nblocks) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  // block occupancy of the new binning starts cleared (set by block_bounds)
-  for (int64_t j = i; j < nblocks; j += (int64_t)gridDim.x * blockDim.x) bocc[j] = 0u;
-  if (i >= n) return;
-  const SceneDev sc = a.scenes[a.s.attr[i] >> 8];
-  const int b0 = cell_of(a.s.x[tix(i, 3, 0)], sc.inv_dx, sc.res);
-  const int b1 = cell_of(a.s.x[tix(i, 3, 1)], sc.inv_dx, sc.res);
-  const int b2 = cell_of(a.s.x[tix(i, 3, 2)], sc.inv_dx, sc.res);
-  const uint32_t blk = (uint32_t)(sc.block_base + ((b0 >> 2) * sc.nb + (b1 >> 2)) * sc.nb + (b2 >> 2));
-  keys[i] = blk * 64u + (uint32_t)(((b0 & 3) << 4) | ((b1 & 3) << 2) | (b2 & 3));
+// Cell keys of the new binning, one sort tile (RS_TILE particles) per
+// workgroup, with the tile histogram of the sort's first digit, so the sort
+// starts at that pass's scan.  Also starts the segment that rebins (the
+// counters' segment_begin) and clears the block occupancy that
+// block_bounds_kernel sets: two launches fewer ahead of the sort.
+__global__ __launch_bounds__(RS_THREADS) void engine_keys_kernel(EngineArgs a, int64_t n,
+                                                                 uint32_t* __restrict__ keys,
+                                                                 uint32_t* __restrict__ bocc, int64_t nblocks,
+                                                                 int len, int with_sums,
+                                                                 uint32_t* __restrict__ hist, int ntiles) {
+  __shared__ uint32_t h[256];
+  const int64_t gid = (int64_t)blockIdx.x * RS_THREADS + threadIdx.x;
+  if (gid == 0) segment_begin(a.counters, len, with_sums, 1);
+  for (int64_t j = gid; j < nblocks; j += (int64_t)gridDim.x * RS_THREADS) bocc[j] = 0u;
+  const bool one_scene = a.nscenes == 1;
+  const SceneDev sc0 = a.scenes[0];
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  // every position of the thread is loaded before the first key is stored
+  // (the key stores could alias the positions as far as the compiler knows,
+  // which would otherwise put one memory round trip per key in series)
+  float px[RS_ITEMS][3];
+  uint32_t sid[RS_ITEMS];
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+    const int64_t j = i < n ? i : 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) px[r][d] = a.s.x[tix(j, 3, d)];
+    sid[r] = one_scene ? 0u : a.s.attr[j] >> 8;
+  }
+  uint32_t k[RS_ITEMS];
+#pragma unroll
+  for (int r = 0; r < RS_ITEMS; ++r) {
+    const int64_t i = base + (int64_t)r * RS_THREADS + threadIdx.x;
+    const SceneDev sc = one_scene ? sc0 : a.scenes[sid[r]];
+    const int b0 = cell_of(px[r][0], sc.inv_dx, sc.res);
+    const int b1 = cell_of(px[r][1], sc.inv_dx, sc.res);
+    const int b2 = cell_of(px[r][2], sc.inv_dx, sc.res);
+    const uint32_t blk = (uint32_t)(sc.block_base + ((b0 >> 2) * sc.nb + (b1 >> 2)) * sc.nb + (b2 >> 2));
+    k[r] = blk * 64u + (uint32_t)(((b0 & 3) << 4) | ((b1 & 3) << 2) | (b2 & 3));
+    if (i < n) keys[i] = k[r];
+  }
+  rs_tile_histogram(k, n, 0, h, hist, ntiles);
 }
 
 __global__ void block_bounds_kernel(const uint32_t* __restrict__ keys, int64_t n,
@@ -1568,17 +1597,19 @@ int Engine::download(mpm_particles* p) {
 // Keys -> stable radix sort -> block ranges -> chunk list -> candidate blocks
 // and their neighbour table.  The particles themselves move into the new
 // order in the segment's first P2G (through pending_perm).
-void Engine::enqueue_bin(hipStream_t s) {
+void Engine::enqueue_bin(hipStream_t s, int len, int with_sums) {
   if (n <= 0) {
+    hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, with_sums, 1);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_nchunks, (int64_t)1, 0u);
     return;
   }
   EngineArgs a = args(nullptr, 0, 0);
-  hipLaunchKernelGGL(engine_keys_kernel, dim3(nblk(n, 256)), dim3(256), 0, s, a, n, d_keys, d_bocc,
-                     (int64_t)total_blocks);
+  const int64_t tiles = radix_tiles(n);
+  hipLaunchKernelGGL(engine_keys_kernel, dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, a, n, d_keys, d_bocc,
+                     (int64_t)total_blocks, len, with_sums, radix_first_hist(d_ws, n), (int)tiles);
   uint32_t *ks, *vs;
   const uint32_t max_key = (uint32_t)(total_blocks * 64 - 1);
-  radix_sort(d_keys, d_vals, n, max_key, d_ws, true, &ks, &vs, s);
+  radix_sort(d_keys, d_vals, n, max_key, d_ws, true, &ks, &vs, s, true);
   // the particles move into binned order in the segment's first P2G, which
   // reads slot perm[p] of the current buffer and writes slot p of the other
   cur ^= 1;
@@ -1671,10 +1702,6 @@ void Engine::enqueue_frame(hipStream_t s, float* frames, int64_t step) {
                      frames + (size_t)step * 3 * (size_t)n);
 }
 
-void Engine::begin_segment(hipStream_t s, int len, int with_sums, int bin) {
-  hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, with_sums, bin);
-}
-
 void Engine::fill_u32(hipStream_t s, uint32_t* p, int64_t count, uint32_t v) {
   hipLaunchKernelGGL(fill_u32_kernel, dim3(nblk(count, 256)), dim3(256), 0, s, p, count, v);
 }
@@ -1689,8 +1716,10 @@ void Engine::enqueue_state(hipStream_t s, const StateSink* sink, int64_t step) {
 void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int max_steps, float* frames,
                              int64_t step0, const StateSink* sink) {
   const int ws = sums ? 1 : 0;
-  hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws, bin ? 1 : 0);
-  if (bin) enqueue_bin(s);
+  if (bin)
+    enqueue_bin(s, len, ws);
+  else
+    hipLaunchKernelGGL(segment_begin_kernel, dim3(1), dim3(64), 0, s, d_counters, len, ws, 0);
   // substep k appends to active list k % 2; its grid update clears the other
   const size_t rec_nn = (size_t)(scenes_host[0].res + 1) * (scenes_host[0].res + 1) * (scenes_host[0].res + 1);
   auto grid_args = [&](int k) {

@@ -11,63 +11,29 @@
 // major exclusive scan of the per-tile histograms.  Each pass is therefore a
 // stable counting sort, and the whole sort equals a stable sort by key.
 #include "mpm_sort.h"
+#include "mpm_sort_dev.h"
 
 namespace mpm {
 
-constexpr int RS_THREADS = 256;
-constexpr int RS_WAVES = RS_THREADS / 64;
-constexpr int RS_ITEMS = 16;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
-
-__device__ __forceinline__ uint64_t lanemask_lt() {
-  const unsigned lane = __lane_id();
-  return lane == 0 ? 0ull : ((~0ull) >> (64 - lane));
-}
-
-// Lanes (among `valid`) whose 8-bit digit equals this lane's.
-__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t valid) {
-  uint64_t m = valid;
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool bit = (d >> b) & 1u;
-    const uint64_t bal = __ballot(bit);
-    m &= bit ? bal : ~bal;
-  }
-  return m;
-}
 
 // Per-tile digit histogram, written digit-major: hist[d * ntiles + tile].
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint32_t* __restrict__ keys,
                                                          int64_t n, int shift,
                                                          uint32_t* __restrict__ hist, int ntiles) {
   __shared__ uint32_t h[256];
-  const int t = threadIdx.x;
-  h[t] = 0;
-  __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-  const unsigned lane = __lane_id();
   // every key of the thread in flight at once: one memory round trip per
   // tile rather than one per few rounds
   uint32_t k[RS_ITEMS];
 #pragma unroll
   for (int r = 0; r < RS_ITEMS; ++r) {
-    const int64_t idx = base + (int64_t)r * RS_THREADS + t;
+    const int64_t idx = base + (int64_t)r * RS_THREADS + threadIdx.x;
     k[r] = idx < n ? keys[idx] : 0u;
   }
-#pragma unroll
-  for (int r = 0; r < RS_ITEMS; ++r) {
-    const int64_t idx = base + (int64_t)r * RS_THREADS + t;
-    const bool valid = idx < n;
-    const uint32_t d = valid ? (k[r] >> shift) & 255u : 0u;
-    const uint64_t vm = __ballot(valid);
-    const uint64_t m = match_digit(d, vm);
-    if (valid && lane == 63u - (unsigned)__clzll(m)) atomicAdd(&h[d], (uint32_t)__popcll(m));
-  }
-  __syncthreads();
-  hist[(int64_t)t * ntiles + blockIdx.x] = h[t];
+  rs_tile_histogram(k, n, shift, h, hist, ntiles);
 }
 
 // Exclusive prefix sum over the 256 threads of a workgroup (one value each):
@@ -315,26 +281,32 @@ int radix_passes(uint32_t max_key) {
   return bits == 0 ? 1 : (bits + 7) / 8;
 }
 
+int64_t radix_tiles(int64_t n) { return cdiv(n > 0 ? n : 1, RS_TILE); }
+
+uint32_t* radix_first_hist(void* ws, int64_t n) { return (uint32_t*)ws + 2 * n; }
+
 // Stable sort of keys[0..n) carrying values; result in (*kout, *vout), which
 // point into either the input arrays or the workspace.  iota_values: the
-// input values are 0..n-1 (not read).
+// input values are 0..n-1 (not read).  first_hist_done: the first pass's
+// tile histogram is already in the workspace (radix_first_hist).
 void radix_sort(uint32_t* keys, uint32_t* vals, int64_t n, uint32_t max_key, void* ws,
-                bool iota_values, uint32_t** kout, uint32_t** vout, hipStream_t st) {
-  const int64_t ntiles = cdiv(n > 0 ? n : 1, RS_TILE);
+                bool iota_values, uint32_t** kout, uint32_t** vout, hipStream_t st, bool first_hist_done) {
+  const int64_t ntiles = radix_tiles(n);
   const int64_t hist_n = 256 * ntiles;
   const int64_t big = hist_n > n ? hist_n : n;
   uint32_t* w = (uint32_t*)ws;
   uint32_t* k2 = w;
   uint32_t* v2 = w + n;
-  uint32_t* hist = w + 2 * n;
+  uint32_t* hist = radix_first_hist(ws, n);
   uint32_t* offs = hist + big;
   uint32_t* dtot = offs + big;
   uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
   const int passes = radix_passes(max_key);
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
-    hipLaunchKernelGGL(rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, n, shift,
-                       hist, (int)ntiles);
+    if (p > 0 || !first_hist_done)
+      hipLaunchKernelGGL(rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, n, shift,
+                         hist, (int)ntiles);
     hipLaunchKernelGGL(rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, st, hist, (int)ntiles, offs, dtot);
     hipLaunchKernelGGL(rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, va, kb, vb,
                        n, shift, offs, dtot, (int)ntiles, (iota_values && p == 0) ? 1 : 0);
