@@ -4,8 +4,8 @@
 //
 // The array holds, for n <= NOALLOC_MAX_N particles: the sort's working copy
 // of the keys (n words), the key/value ping-pong buffers (2n), and the
-// per-tile digit histograms with their offsets (2 * max(256 * tiles, n)) and
-// the 256 digit totals.  Each GPU holds its own copy for the life of the
+// per-tile digit histograms with their offsets (2 * max(RS_BINS * tiles, n))
+// and the RS_BINS digit totals.  Each GPU holds its own copy for the life of the
 // process; its address is looked up once per device (hipGetSymbolAddress)
 // and only that address is handed to kernels.  Nothing in it needs to start
 // at zero: every word a call reads was written earlier in the same call.
@@ -15,15 +15,16 @@
 
 #include "mpm355.h"
 #include "mpm_sort.h"
+#include "mpm_sort_dev.h"
 
 namespace mpm {
 void set_error(const std::string& msg);
 
 constexpr int64_t NOALLOC_MAX_N = int64_t(1) << 20;
-constexpr int64_t NOALLOC_TILE = 4096;  // keys per radix tile (mpm_sort.hip)
-constexpr int64_t NOALLOC_TILES = (NOALLOC_MAX_N + NOALLOC_TILE - 1) / NOALLOC_TILE;
-constexpr int64_t NOALLOC_BIG = 256 * NOALLOC_TILES > NOALLOC_MAX_N ? 256 * NOALLOC_TILES : NOALLOC_MAX_N;
-constexpr int64_t NOALLOC_WORDS = NOALLOC_MAX_N + 2 * NOALLOC_MAX_N + 2 * NOALLOC_BIG + 256 + 64;
+constexpr int64_t NOALLOC_TILES = (NOALLOC_MAX_N + RS_TILE - 1) / RS_TILE;
+constexpr int64_t NOALLOC_BIG =
+    RS_BINS * NOALLOC_TILES > NOALLOC_MAX_N ? RS_BINS * NOALLOC_TILES : NOALLOC_MAX_N;
+constexpr int64_t NOALLOC_WORDS = NOALLOC_MAX_N + 2 * NOALLOC_MAX_N + 2 * NOALLOC_BIG + RS_BINS + 64;
 static_assert(NOALLOC_WORDS * 4 <= (int64_t(24) << 20), "scratch stays under 24 MiB");
 
 static __device__ __align__(256) uint32_t scratch[NOALLOC_WORDS];

@@ -1159,8 +1159,8 @@ __global__ __launch_bounds__(RS_THREADS) void engine_keys_kernel(EngineArgs a, i
                                                                  uint32_t* __restrict__ keys,
                                                                  uint32_t* __restrict__ bocc, int64_t nblocks,
                                                                  int len, int with_sums,
-                                                                 uint32_t* __restrict__ hist, int ntiles) {
-  __shared__ uint32_t h[256];
+                                                                 uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[RS_BINS];
   const int64_t gid = (int64_t)blockIdx.x * RS_THREADS + threadIdx.x;
   if (gid == 0) segment_begin(a.counters, len, with_sums, 1);
   for (int64_t j = gid; j < nblocks; j += (int64_t)gridDim.x * RS_THREADS) bocc[j] = 0u;
@@ -1192,7 +1192,7 @@ __global__ __launch_bounds__(RS_THREADS) void engine_keys_kernel(EngineArgs a, i
     k[r] = blk * 64u + (uint32_t)(((b0 & 3) << 4) | ((b1 & 3) << 2) | (b2 & 3));
     if (i < n) keys[i] = k[r];
   }
-  rs_tile_histogram(k, n, 0, h, hist, ntiles);
+  rs_tile_histogram(k, n, 0, h, hist);
 }
 
 __global__ void block_bounds_kernel(const uint32_t* __restrict__ keys, int64_t n,
@@ -1606,7 +1606,7 @@ void Engine::enqueue_bin(hipStream_t s, int len, int with_sums) {
   EngineArgs a = args(nullptr, 0, 0);
   const int64_t tiles = radix_tiles(n);
   hipLaunchKernelGGL(engine_keys_kernel, dim3((unsigned)tiles), dim3(RS_THREADS), 0, s, a, n, d_keys, d_bocc,
-                     (int64_t)total_blocks, len, with_sums, radix_first_hist(d_ws, n), (int)tiles);
+                     (int64_t)total_blocks, len, with_sums, radix_first_hist(d_ws, n));
   uint32_t *ks, *vs;
   const uint32_t max_key = (uint32_t)(total_blocks * 64 - 1);
   radix_sort(d_keys, d_vals, n, max_key, d_ws, true, &ks, &vs, s, true);

@@ -1,15 +1,16 @@
 // Cell binning: key generation, a stable LSD radix sort on 32-bit keys
-// (8-bit digits; three launches per pass: tile histograms, a per-digit scan
+// (11-bit digits; three launches per pass: tile histograms, a per-digit scan
 // over tiles, the stable scatter), and the chunk list (a two-launch scan of
 // per-block chunk counts) that gives the engine its work items for the
 // transfer kernels.
 //
 // Stability: inside a tile each wave owns a contiguous run of keys and walks
 // it in index order; ranks among equal digits come from a 64-lane match mask
-// built with eight ballots plus per-wave running counters, and waves are
-// ordered by a prefix over their counters.  Tiles are ordered by the digit-
-// major exclusive scan of the per-tile histograms.  Each pass is therefore a
-// stable counting sort, and the whole sort equals a stable sort by key.
+// built with one ballot per digit bit plus per-wave running counters, and
+// waves are ordered by a prefix over their counters.  Tiles are ordered by
+// the per-digit exclusive scan over tiles of the per-tile histograms.  Each
+// pass is therefore a stable counting sort, and the whole sort equals a
+// stable sort by key.
 #include "mpm_sort.h"
 #include "mpm_sort_dev.h"
 
@@ -19,11 +20,11 @@ constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 16;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
 
-// Per-tile digit histogram, written digit-major: hist[d * ntiles + tile].
+// Per-tile digit histogram, written tile-major: hist[tile * RS_BINS + d].
 __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint32_t* __restrict__ keys,
                                                          int64_t n, int shift,
-                                                         uint32_t* __restrict__ hist, int ntiles) {
-  __shared__ uint32_t h[256];
+                                                         uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[RS_BINS];
   const int64_t base = (int64_t)blockIdx.x * RS_TILE;
   // every key of the thread in flight at once: one memory round trip per
   // tile rather than one per few rounds
@@ -33,7 +34,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_upsweep(const uint32_t* __restr
     const int64_t idx = base + (int64_t)r * RS_THREADS + threadIdx.x;
     k[r] = idx < n ? keys[idx] : 0u;
   }
-  rs_tile_histogram(k, n, shift, h, hist, ntiles);
+  rs_tile_histogram(k, n, shift, h, hist);
 }
 
 // Exclusive prefix sum over the 256 threads of a workgroup (one value each):
@@ -55,44 +56,74 @@ __device__ __forceinline__ uint32_t excl_scan_256(uint32_t v, uint32_t* wave_tot
   return base + incl - v;
 }
 
-// One workgroup per digit d: the exclusive prefix of hist[d][*] over tiles
-// (written to offs[d][*]) and the digit's total (dtot[d]).  The scatter adds
-// the exclusive prefix of the totals over digits.
-__global__ __launch_bounds__(RS_THREADS) void rs_digit_scan(const uint32_t* __restrict__ hist, int ntiles,
-                                                           uint32_t* __restrict__ offs,
-                                                           uint32_t* __restrict__ dtot) {
-  __shared__ uint32_t wt[RS_WAVES];
-  const int64_t row = (int64_t)blockIdx.x * ntiles;
-  uint32_t carry = 0;
-  for (int b0 = 0; b0 < ntiles; b0 += RS_THREADS) {
-    const int b = b0 + (int)threadIdx.x;
-    const uint32_t v = b < ntiles ? hist[row + b] : 0u;
-    const uint32_t ex = excl_scan_256(v, wt);
-    if (b < ntiles) offs[row + b] = carry + ex;
-    uint32_t tot = 0;
+// Per digit d: the exclusive prefix over tiles of the tile-major histogram
+// hist[tile * RS_BINS + d] (written to offs in the same layout) and the
+// digit's total (dtot[d]); the scatter adds the exclusive prefix of the
+// totals over digits.  A workgroup takes 64 digits (lane = digit, so every
+// read and write is one contiguous 256-byte row piece) and splits the tiles
+// into DS_WAVES contiguous ranges, one per wave: each wave sums its range,
+// the waves' sums are prefixed in LDS, and each wave then writes its range's
+// running prefix.  Sixteen waves keep each wave's serial walk short (16
+// tiles for 2^20 keys).
+constexpr int DS_WAVES = 16;
+__global__ __launch_bounds__(DS_WAVES * 64) void rs_digit_scan(const uint32_t* __restrict__ hist, int ntiles,
+                                                              uint32_t* __restrict__ offs,
+                                                              uint32_t* __restrict__ dtot) {
+  __shared__ uint32_t part[DS_WAVES][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int d = blockIdx.x * 64 + lane;
+  const int per = (ntiles + DS_WAVES - 1) / DS_WAVES;
+  const int t0 = w * per < ntiles ? w * per : ntiles;
+  const int t1 = t0 + per < ntiles ? t0 + per : ntiles;
+  uint32_t sum = 0;
+#pragma unroll 16
+  for (int tl = t0; tl < t1; ++tl) sum += hist[(int64_t)tl * RS_BINS + d];
+  part[w][lane] = sum;
+  __syncthreads();
+  uint32_t run = 0;
 #pragma unroll
-    for (int i = 0; i < RS_WAVES; ++i) tot += wt[i];
-    carry += tot;
-    __syncthreads();  // wt is rewritten by the next round
+  for (int i = 0; i < DS_WAVES; ++i) run += i < w ? part[i][lane] : 0u;
+  if (w == DS_WAVES - 1) dtot[d] = run + sum;
+#pragma unroll 16
+  for (int tl = t0; tl < t1; ++tl) {
+    const uint32_t v = hist[(int64_t)tl * RS_BINS + d];
+    offs[(int64_t)tl * RS_BINS + d] = run;
+    run += v;
   }
-  if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
 __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int64_t n, int shift, const uint32_t* __restrict__ offs,
     const uint32_t* __restrict__ dtot, int ntiles, int iota_values) {
-  __shared__ uint32_t cnt[RS_WAVES][256];
-  __shared__ uint32_t goff[256];
+  __shared__ uint32_t cnt[RS_WAVES][RS_BINS];
+  __shared__ uint32_t goff[RS_BINS];
   __shared__ uint32_t wt[RS_WAVES];
   const int t = threadIdx.x;
   const int w = t >> 6;
   const unsigned lane = __lane_id();
+  constexpr int DPT = RS_BINS / RS_THREADS;  // digits per thread
 #pragma unroll
-  for (int i = 0; i < RS_WAVES; ++i) cnt[i][t] = 0;
-  // global start of digit t in this tile: keys of smaller digits (all tiles)
-  // plus keys of digit t in earlier tiles
-  goff[t] = excl_scan_256(dtot[t], wt) + offs[(int64_t)t * ntiles + blockIdx.x];
+  for (int i = 0; i < RS_WAVES; ++i)
+    for (int d = t; d < RS_BINS; d += RS_THREADS) cnt[i][d] = 0;
+  // global start of digit d in this tile: keys of smaller digits (all tiles)
+  // plus keys of digit d in earlier tiles; thread t takes digits
+  // t * DPT .. t * DPT + DPT - 1
+  {
+    uint32_t tot[DPT], sum = 0;
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+      tot[j] = dtot[t * DPT + j];
+      sum += tot[j];
+    }
+    uint32_t run = excl_scan_256(sum, wt);
+#pragma unroll
+    for (int j = 0; j < DPT; ++j) {
+      const int d = t * DPT + j;
+      goff[d] = run + offs[(int64_t)blockIdx.x * RS_BINS + d];
+      run += tot[j];
+    }
+  }
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * RS_TILE + (int64_t)w * 64 * RS_ITEMS;
   uint32_t k[RS_ITEMS], val[RS_ITEMS];
@@ -110,7 +141,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
   for (int r = 0; r < RS_ITEMS; ++r) {
     const int64_t idx = base + r * 64 + lane;
     const bool valid = idx < n;
-    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint32_t d = (k[r] >> shift) & (uint32_t)(RS_BINS - 1);
     const uint64_t vm = __ballot(valid);
     const uint64_t m = match_digit(d, vm);
     uint32_t prev = 0;
@@ -119,12 +150,12 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
     if (valid && lane == 63u - (unsigned)__clzll(m)) cnt[w][d] = prev + (uint32_t)__popcll(m);
   }
   __syncthreads();
-  {
+  for (int d = t; d < RS_BINS; d += RS_THREADS) {
     uint32_t run = 0;
 #pragma unroll
     for (int i = 0; i < RS_WAVES; ++i) {
-      const uint32_t c = cnt[i][t];
-      cnt[i][t] = run;
+      const uint32_t c = cnt[i][d];
+      cnt[i][d] = run;
       run += c;
     }
   }
@@ -133,7 +164,7 @@ __global__ __launch_bounds__(RS_THREADS) void rs_downsweep(
   for (int r = 0; r < RS_ITEMS; ++r) {
     const int64_t idx = base + r * 64 + lane;
     if (idx < n) {
-      const uint32_t d = (k[r] >> shift) & 255u;
+      const uint32_t d = (k[r] >> shift) & (uint32_t)(RS_BINS - 1);
       const uint32_t dst = goff[d] + cnt[w][d] + rank[r];
       kout[dst] = k[r];
       vout[dst] = val[r];
@@ -256,10 +287,9 @@ static inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 size_t radix_workspace_bytes(int64_t n) {
   const int64_t ntiles = cdiv(n > 0 ? n : 1, RS_TILE);
-  const int64_t hist = 256 * ntiles;
-  const int64_t parts = cdiv(hist > n ? hist : n, SCAN_TILE) + 1;
-  // keys/vals ping-pong (4 arrays of n) + hist + offs + partials + digit totals
-  return sizeof(uint32_t) * (4 * (size_t)n + 2 * (size_t)(hist > n ? hist : n) + parts + 256 + 64);
+  const int64_t hist = (int64_t)RS_BINS * ntiles;
+  // keys/vals ping-pong (4 arrays of n) + hist + offs + digit totals
+  return sizeof(uint32_t) * (4 * (size_t)n + 2 * (size_t)(hist > n ? hist : n) + RS_BINS + 64);
 }
 
 size_t chunk_list_part_words(int64_t nblocks) { return (size_t)cdiv(nblocks, SCAN_THREADS) + 1; }
@@ -278,7 +308,7 @@ void chunk_list(const uint32_t* bocc, const uint32_t* bstart, const uint32_t* be
 int radix_passes(uint32_t max_key) {
   int bits = 0;
   while (bits < 32 && (max_key >> bits) != 0u) ++bits;
-  return bits == 0 ? 1 : (bits + 7) / 8;
+  return bits == 0 ? 1 : (bits + RS_BITS - 1) / RS_BITS;
 }
 
 int64_t radix_tiles(int64_t n) { return cdiv(n > 0 ? n : 1, RS_TILE); }
@@ -292,7 +322,7 @@ uint32_t* radix_first_hist(void* ws, int64_t n) { return (uint32_t*)ws + 2 * n; 
 void radix_sort(uint32_t* keys, uint32_t* vals, int64_t n, uint32_t max_key, void* ws,
                 bool iota_values, uint32_t** kout, uint32_t** vout, hipStream_t st, bool first_hist_done) {
   const int64_t ntiles = radix_tiles(n);
-  const int64_t hist_n = 256 * ntiles;
+  const int64_t hist_n = (int64_t)RS_BINS * ntiles;
   const int64_t big = hist_n > n ? hist_n : n;
   uint32_t* w = (uint32_t*)ws;
   uint32_t* k2 = w;
@@ -303,11 +333,11 @@ void radix_sort(uint32_t* keys, uint32_t* vals, int64_t n, uint32_t max_key, voi
   uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
   const int passes = radix_passes(max_key);
   for (int p = 0; p < passes; ++p) {
-    const int shift = 8 * p;
+    const int shift = RS_BITS * p;
     if (p > 0 || !first_hist_done)
-      hipLaunchKernelGGL(rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, n, shift,
-                         hist, (int)ntiles);
-    hipLaunchKernelGGL(rs_digit_scan, dim3(256), dim3(RS_THREADS), 0, st, hist, (int)ntiles, offs, dtot);
+      hipLaunchKernelGGL(rs_upsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, n, shift, hist);
+    hipLaunchKernelGGL(rs_digit_scan, dim3(RS_BINS / 64), dim3(DS_WAVES * 64), 0, st, hist, (int)ntiles, offs,
+                       dtot);
     hipLaunchKernelGGL(rs_downsweep, dim3((unsigned)ntiles), dim3(RS_THREADS), 0, st, ka, va, kb, vb,
                        n, shift, offs, dtot, (int)ntiles, (iota_values && p == 0) ? 1 : 0);
     uint32_t* t;
