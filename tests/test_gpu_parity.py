@@ -31,7 +31,9 @@ def _say(*a):
 # ------------------------------------------------------------- binning ----
 @pytest.mark.parametrize("n,res,layout", [(1, 8, "blocked"), (4095, 32, "blocked"), (4097, 32, "rowmajor"),
                                           (100_000, 128, "blocked"), (100_000, 128, "rowmajor"),
-                                          (300_001, 100, "blocked")])
+                                          (300_001, 100, "blocked"),
+                                          # keys past 22 bits: three 11-bit radix passes
+                                          (200_003, 256, "blocked"), (60_000, 300, "rowmajor")])
 def test_bin_bit_exact(gpu, n, res, layout):
     from femflow_amd.engine import bin_particles
 
