@@ -416,3 +416,22 @@ def test_engine_full_size_one_substep_vs_c_oracle(gpu):
         assert errs[k] < 1e-5, errs
     assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
     eng.close()
+
+
+def test_engine_256_grid_vs_c_oracle(gpu):
+    """A 256^3 grid: 65^3 blocks, cell keys of 25 bits (three radix passes in
+    the engine's own binning) and block indices past 2^18; three
+    three-material cubes, 4 substeps with a rebin every 2, against the C
+    restatement (FP64)."""
+    st, pr = O.scene_3d_cubes(n=30000, res=256, seed=41, side=0.08)
+    ts = to_torch_state(st, gpu)
+    ref_in = oracle_copy_from(ts, st)
+    eng = engine_for(st, pr, gpu, sort_every=2)
+    out = engine_roundtrip(eng, ts, 4)
+    ref = CO.run(ref_in, pr, 4, nthreads=8).to_state()
+    errs = field_errors(out, ref, pr["inv_dx"])
+    _say("256 grid errs", errs)
+    for k in ("x", "v", "F"):
+        assert errs[k] < 1e-5, errs
+    assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
+    eng.close()
