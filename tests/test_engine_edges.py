@@ -290,3 +290,40 @@ def test_engine_high_walls(gpu):
         assert errs[k] < 1e-5, errs
     assert errs["C"] < 1e-4 and errs["Jp"] < 1e-5, errs
     eng.close()
+
+
+def test_engine_run_splits_are_invariant(gpu):
+    """The same 60 substeps as one run, as two runs of 30 and as eight runs of
+    7 plus one of 4.  The rebin schedule (every 16 substeps) counts across
+    runs, so only the run boundaries differ: a split run ends with a G2P that
+    stores v and C and starts with a P2G that reads them back, where one run
+    keeps them in registers through the fused pass.  With no FP64-deferred
+    particles every split gives the same state to FP32 rounding: the
+    fixed-point arena sums do not depend on the order of the scatter, and
+    only the few contributions that reach a node through global float
+    atomics (a particle that crossed a block face since its binning) may
+    add in another order."""
+    st, pr = O.scene_3d_cubes(n=20000, res=64, seed=51)
+    ts = to_torch_state(st, gpu)
+    eng = engine_for(st, pr, gpu, sort_every=16)
+    outs = []
+    for splits in ([60], [30, 30], [7] * 8 + [4]):
+        eng.upload(ts["x"], ts["v"], ts["F"], ts["C"], ts["Jp"], ts["mass"], ts["mu0"], ts["lam0"],
+                   ts["kind"])
+        eng.fallback_count()  # reading resets it
+        for k in splits:
+            eng.run(k)
+        out = {k: torch.empty_like(ts[k]) for k in ("x", "v", "F", "C", "Jp")}
+        eng.download(out["x"], out["v"], out["F"], out["C"], out["Jp"])
+        torch.cuda.synchronize()
+        fb = eng.fallback_count()
+        _say("splits", splits, "fallbacks", fb)
+        assert fb == 0  # the comparison below is of the FP32 path only
+        outs.append(out)
+    for other in outs[1:]:
+        for k in ("x", "v", "F", "C", "Jp"):
+            a, b = outs[0][k].double(), other[k].double()
+            d = float((a - b).abs().max()) / max(float(a.abs().max()), 1e-30)
+            _say("split difference", k, d)
+            assert d <= 1e-6, (k, d)
+    eng.close()
