@@ -15,16 +15,17 @@ namespace mpm {
 void set_error(const std::string& msg);
 
 // A captured segment fixes the particle count (launch grids, sort sizes),
-// the segment length, whether it rebins first, which state buffer is current
-// and the sums destination; any of them changing needs another graph.
+// the segment length, whether it rebins first, which state buffer is current,
+// the sums destination, and whether it continues the run's previous segment
+// (cont) and ends the run (fin); any of them changing needs another graph.
 struct GraphKey {
   int64_t n;
   int len, parity, bin;
   uintptr_t sums;
-  int nsteps;
+  int nsteps, cont, fin;
   bool operator<(const GraphKey& o) const {
-    return std::tie(n, len, parity, bin, sums, nsteps) <
-           std::tie(o.n, o.len, o.parity, o.bin, o.sums, o.nsteps);
+    return std::tie(n, len, parity, bin, sums, nsteps, cont, fin) <
+           std::tie(o.n, o.len, o.parity, o.bin, o.sums, o.nsteps, o.cont, o.fin);
   }
 };
 // Captured graphs kept per context; beyond this the cache is emptied first.
@@ -126,8 +127,8 @@ struct Engine {
   int download(mpm_particles* p);
   // rebin; also starts the segment of len substeps (counters, sums index)
   void enqueue_bin(hipStream_t s, int len, int with_sums);
-  void enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int max_steps, float* frames = nullptr,
-                       int64_t step0 = 0, const StateSink* sink = nullptr);
+  void enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int max_steps, float* frames,
+                       int64_t step0, const StateSink* sink, bool cont, bool fin);
   // destroy every captured graph (after anything they baked in changes)
   void drop_graphs();
   int active_blocks(int64_t* out);

@@ -418,6 +418,10 @@ __device__ __forceinline__ int round_fixed(float w, float q) {
 // runtime row index (a few VALU per slab).  Measured at 2^19 particles:
 // equal at 16 particles per cell, 25% / 34% faster at 128 / 1,157, and 1.5%
 // faster on the three-cube 2^20 scene (~21 per cell), so 9 is the default.
+// MODE: 0 = P2G (the first substep of a run; after a rebin it also moves
+// the particles into the new binned order), 1 = fused G2P + P2G, 2 = G2P
+// (the last substep of a run), 3 = the fused pass that also moves the
+// particles (the first substep after a rebin inside a run).
 // DYN: workgroups take their chunks from a per-XCD ticket queue instead of
 // a fixed interleave.  With tens of thousands of uneven chunks (many scenes
 // of different densities) the queue balances the tail; with a few thousand
@@ -508,24 +512,41 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
       float pmass, pmu0, plam0;
       int base[3];
       uint32_t attr;
-      if (MODE == 0 && a.perm) {
+      // MODE 3: the fused pass of a rebin inside a run; binned slot p
+      // takes the particle of slot perm[p] of the previous order, whose G2P
+      // inputs are read from there and whose constants are copied here
+      const int64_t ps = MODE == 3 ? (int64_t)a.perm[p] : p;
+      if constexpr (MODE == 3) {
+        loadk<3>(a.src.x, ps, x);
+        loadk<9>(a.src.F, ps, F);
+        Jp = a.src.Jp[ps];
+        attr = a.src.attr[ps];
+        pmass = a.src.mass[ps];
+        pmu0 = a.src.mu0[ps];
+        plam0 = a.src.lam0[ps];
+        a.s.attr[p] = attr;
+        a.s.mass[p] = pmass;
+        a.s.mu0[p] = pmu0;
+        a.s.lam0[p] = plam0;
+        a.s.order[p] = a.src.order[ps];
+      } else if (MODE == 0 && a.perm) {
         // binned slot p takes the particle of slot perm[p] of the previous
         // order: every field the fused pass reads is copied here, before any
         // early exit.  v and C are read by the P2G below from registers and
         // recomputed by the next G2P, so the new slot gets them only where
         // they are read before that (FP64-deferred particles, per-substep
         // state output) or kept (particles that left the grid).
-        const int64_t ps = a.perm[p];
-        loadk<3>(a.src.x, ps, x);
-        loadk<9>(a.src.F, ps, F);
-        loadk<3>(a.src.v, ps, v);
-        loadk<9>(a.src.C, ps, C);
-        Jp = a.src.Jp[ps];
-        attr = a.src.attr[ps];
-        pmass = a.src.mass[ps];
-        pmu0 = a.src.mu0[ps];
-        plam0 = a.src.lam0[ps];
-        const uint32_t ord = a.src.order[ps];
+        const int64_t pm = a.perm[p];
+        loadk<3>(a.src.x, pm, x);
+        loadk<9>(a.src.F, pm, F);
+        loadk<3>(a.src.v, pm, v);
+        loadk<9>(a.src.C, pm, C);
+        Jp = a.src.Jp[pm];
+        attr = a.src.attr[pm];
+        pmass = a.src.mass[pm];
+        pmu0 = a.src.mu0[pm];
+        plam0 = a.src.lam0[pm];
+        const uint32_t ord = a.src.order[pm];
         storek<3>(a.s.x, p, x);
         storek<9>(a.s.F, p, F);
         if (a.store_vc) {
@@ -556,6 +577,15 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
       const MatParams mp = a.mats[mid];
       if constexpr (MODE != 0) {
         if (!locate3(x, sc, base, fx)) {
+          if constexpr (MODE == 3) {  // the particle keeps its state in the new slot
+            storek<3>(a.s.x, p, x);
+            storek<9>(a.s.F, p, F);
+            a.s.Jp[p] = Jp;
+            loadk<3>(a.src.v, ps, v);
+            loadk<9>(a.src.C, ps, C);
+            storek<3>(a.s.v, p, v);
+            storek<9>(a.s.C, p, C);
+          }
           atomicAdd(&wg_oob, 1u);
           continue;
         }
@@ -599,7 +629,7 @@ __global__ __launch_bounds__(ENG_THREADS) void transfer_kernel(EngineArgs a) {
         for (int d = 0; d < 3; ++d) bspline(fx[d], w[d]);
         P2GTerms tm;
         if (!p2g_terms(sc, mp, pmass, pmu0, plam0, Jp, F, C, v, fx, tm)) {
-          if ((MODE == 1 && !a.store_vc) || vc_unstored) {  // the FP64 pass reads them
+          if (((MODE == 1 || MODE == 3) && !a.store_vc) || vc_unstored) {  // the FP64 pass reads them
             storek<3>(a.s.v, p, v);
             storek<9>(a.s.C, p, C);
           }
@@ -1651,19 +1681,24 @@ void Engine::launch_transfer(int mode, hipStream_t s, const EngineArgs& a, bool 
   } else if (chunk == 256) {
     if (mode == 0) MPM_LAUNCH_T(0, 1);
     else if (mode == 1) MPM_LAUNCH_T(1, 1);
-    else MPM_LAUNCH_T(2, 1);
+    else if (mode == 2) MPM_LAUNCH_T(2, 1);
+    else MPM_LAUNCH_T(3, 1);
   } else if (chunk == 512) {
     if (mode == 0) MPM_LAUNCH_T(0, 2);
     else if (mode == 1) MPM_LAUNCH_T(1, 2);
-    else MPM_LAUNCH_T(2, 2);
+    else if (mode == 2) MPM_LAUNCH_T(2, 2);
+    else MPM_LAUNCH_T(3, 2);
   } else {
     if (mode == 0) MPM_LAUNCH_T(0, 4);
     else if (mode == 1) MPM_LAUNCH_T(1, 4);
-    else MPM_LAUNCH_T(2, 4);
+    else if (mode == 2) MPM_LAUNCH_T(2, 4);
+    else MPM_LAUNCH_T(3, 4);
   }
 #undef MPM_LAUNCH_T
 #undef MPM_LAUNCH_K
-  if (timing) tpairs.push_back({mode, e0, e1});
+  // timing slots: 0 the segment-start pass (P2G, or the moving fused pass),
+  // 1 fused, 2 G2P (3 is the grid update's)
+  if (timing) tpairs.push_back({mode == 3 ? 0 : mode, e0, e1});
   // one workgroup per CU: the usual empty list costs each workgroup one load,
   // and a violent step that defers tens of thousands of particles to FP64
   // (seen with dt several times the stable step) is spread over the chip
@@ -1713,8 +1748,13 @@ void Engine::enqueue_state(hipStream_t s, const StateSink* sink, int64_t step) {
                      sink->v + 3 * o, sink->F + 9 * o, sink->C + 9 * o, sink->Jp + o);
 }
 
+// cont: the run's previous segment ended after its last grid update (no
+// G2P), and this one rebins; its first pass is then the fused G2P of that
+// substep plus this segment's first P2G, moving the particles into the new
+// order (MODE 3) instead of a G2P, the rebin and a moving P2G.  fin: the
+// run's last segment, which ends with the G2P.
 void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int max_steps, float* frames,
-                             int64_t step0, const StateSink* sink) {
+                             int64_t step0, const StateSink* sink, bool cont, bool fin) {
   const int ws = sums ? 1 : 0;
   if (bin)
     enqueue_bin(s, len, ws);
@@ -1736,7 +1776,11 @@ void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int
     EngineArgs a0 = args(sums, max_steps, 0);
     a0.perm = bin ? pending_perm : nullptr;
     pending_perm = nullptr;
-    launch_transfer(0, s, a0, sep);
+    launch_transfer(cont ? 3 : 0, s, a0, sep);
+  }
+  if (cont) {  // the previous segment's last substep finished its G2P
+    enqueue_frame(s, frames, step0 - 1);
+    enqueue_state(s, sink, step0 - 1);
   }
   if (corr_on) launch_cellf(s, args(sums, max_steps, 0));
   launch_grid(s, grid_args(0), ws, len, 0, sep ? 0 : 1);
@@ -1749,9 +1793,11 @@ void Engine::enqueue_segment(int len, bool bin, hipStream_t s, double* sums, int
     launch_grid(s, grid_args(k), ws, len, k, sep ? 0 : 1);
     if (corr_on) launch_corrector(s, args(sums, max_steps, k & 1), nullptr);
   }
-  launch_transfer(2, s, args(sums, max_steps, len & 1));
-  enqueue_frame(s, frames, step0 + len - 1);
-  enqueue_state(s, sink, step0 + len - 1);
+  if (fin) {
+    launch_transfer(2, s, args(sums, max_steps, len & 1));
+    enqueue_frame(s, frames, step0 + len - 1);
+    enqueue_state(s, sink, step0 + len - 1);
+  }
   store_vc = false;
 }
 
@@ -1779,23 +1825,33 @@ int Engine::run(int nsteps, double* sums, float* frames, const StateSink* sink) 
   // carries over from one call to the next, so callers that advance a few
   // substeps per call do not pay a rebin per call.  Every call ends with
   // complete particle state (G2P), so a call costs one split P2G / G2P pair
-  // beyond the fused substeps.
+  // beyond the fused substeps.  Inside a call a segment ends after its last
+  // grid update and the next one (which always rebins: segments are cut at
+  // the schedule's rebins) starts with the fused G2P of that substep, which
+  // also moves the particles into the new order.
   int done = 0;
   while (done < nsteps) {
     const bool bin = since_bin < 0 || since_bin >= sort_every || n <= 0;
     const int room = bin ? sort_every : sort_every - since_bin;
     const int len = (nsteps - done) < room ? (nsteps - done) : room;
+    const bool cont = n > 0 && done > 0;
+    const bool fin = n <= 0 || done + len == nsteps;
+    if (cont && !bin) {
+      set_error("mpm_run: internal error: a continued segment must rebin");
+      return MPM_ERR_HIP;
+    }
     const bool graphed = use_graph && !timing && n > 0 && !frames && !sink;
     if (graphed) {
       const int parity = cur;
-      const GraphKey key{n, len, parity, bin ? 1 : 0, (uintptr_t)sums, sums ? nsteps : 0};
+      const GraphKey key{n, len, parity, bin ? 1 : 0, (uintptr_t)sums, sums ? nsteps : 0, cont ? 1 : 0,
+                         fin ? 1 : 0};
       auto it = graphs.find(key);
       hipGraphExec_t exec = nullptr;
       if (it == graphs.end()) {
         if (graphs.size() >= MAX_GRAPHS) drop_graphs();
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(own_stream, hipStreamCaptureModeThreadLocal));
-        enqueue_segment(len, bin, own_stream, sums, nsteps);
+        enqueue_segment(len, bin, own_stream, sums, nsteps, nullptr, 0, nullptr, cont, fin);
         HIPCHK(hipStreamEndCapture(own_stream, &graph));
         HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
         HIPCHK(hipGraphDestroy(graph));
@@ -1807,7 +1863,7 @@ int Engine::run(int nsteps, double* sums, float* frames, const StateSink* sink) 
       HIPCHK(hipGraphLaunch(exec, own_stream));
       cur = bin ? parity ^ 1 : parity;
     } else {
-      enqueue_segment(len, bin, own_stream, sums, nsteps, frames, done, sink);
+      enqueue_segment(len, bin, own_stream, sums, nsteps, frames, done, sink, cont, fin);
     }
     HIPCHK(hipGetLastError());
     since_bin = bin ? len : since_bin + len;
