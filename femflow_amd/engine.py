@@ -289,7 +289,9 @@ class Engine:
         _lib.check(self._L.mpm_ctx_kernel_timing(self._h, 1 if enable else 0), "kernel_timing")
 
     def kernel_times(self):
-        """{name: (total_ms, launches)} since the previous call (timing mode)."""
+        """{name: (total_ms, launches)} since the previous call (timing mode).
+        "p2g" is the pass that starts a segment: a run's first P2G, or, after
+        a rebin inside a run, the fused pass that moves the particles."""
         ms = (ct.c_double * 4)()
         cnt = (ct.c_int64 * 4)()
         _lib.check(self._L.mpm_ctx_kernel_times(self._h, ms, cnt), "kernel_times")

@@ -160,8 +160,9 @@ int32_t mpm_ctx_set_spread(mpm_ctx* ctx, int32_t spread);
 
 /* Per-kernel timing with HIP events on the engine's stream (eager launches
  * only; graphs are not used while enabled).  mpm_ctx_kernel_times returns
- * total milliseconds and launch counts for [P2G, fused G2P2G, G2P, grid
- * update] since the previous call. */
+ * total milliseconds and launch counts for [segment-start pass (a run's
+ * first P2G, or the fused pass that moves the particles after a rebin inside
+ * a run), fused G2P2G, G2P, grid update] since the previous call. */
 int32_t mpm_ctx_kernel_timing(mpm_ctx* ctx, int32_t enable);
 int32_t mpm_ctx_kernel_times(mpm_ctx* ctx, double* ms, int64_t* counts);
 
