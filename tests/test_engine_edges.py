@@ -110,6 +110,21 @@ def test_engine_run_states_matches_frames_and_download(gpu):
     assert torch.equal(v[-1], last["v"]) and torch.equal(F[-1], last["F"])
     assert torch.equal(C[-1], last["C"]) and torch.equal(Jp[-1], last["Jp"])
     assert not torch.equal(x[0], x[-1])
+    # each recorded substep against a run that ends there: rebins every 3
+    # substeps put the G2P of substeps 2 and 5 into the fused pass that moves
+    # the particles, whose per-substep state output is checked here
+    eng = engine_for(st, pr, gpu, sort_every=3)
+    for j in range(k):
+        eng.upload(ts["x"], ts["v"], ts["F"], ts["C"], ts["Jp"], ts["mass"], ts["mu0"], ts["lam0"], ts["kind"])
+        eng.run(j + 1)
+        got = {f: torch.empty_like(ts[f]) for f in ("x", "v", "F", "C", "Jp")}
+        eng.download(got["x"], got["v"], got["F"], got["C"], got["Jp"])
+        torch.cuda.synchronize()
+        for f, rec in (("x", x), ("v", v), ("F", F), ("C", C), ("Jp", Jp)):
+            a, b = rec[j].reshape(got[f].shape).double(), got[f].double()
+            d = float((a - b).abs().max()) / max(float(b.abs().max()), 1e-30)
+            assert d <= 1e-6, (j, f, d)
+    eng.close()
 
 
 def _rotation(rng):
